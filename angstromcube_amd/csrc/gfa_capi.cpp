@@ -1,0 +1,292 @@
+// C-ABI entry points (include/gfa.h).  Every entry point catches all C++ exceptions and reports
+// failures through a status code and gfa_last_error(); none terminates the process.
+#include "../../include/gfa.h"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "gfa_device.hpp"
+#include "gfa_solver.hpp"
+
+struct gfa_plan {
+    gfa::Plan host;
+    gfa::DevicePlan dev;
+    bool values_dirty = true;
+    bool energy_set = false;
+    int echo = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, std::string const& msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        g_err.clear();
+        return f();
+    } catch (std::bad_alloc const&) {
+        return fail(GFA_ERR_MEMORY, "host allocation failed");
+    } catch (std::exception const& e) {
+        return fail(GFA_ERR_ARGUMENT, e.what());
+    } catch (...) {
+        return fail(GFA_ERR_ARGUMENT, "unknown exception");
+    }
+}
+
+int ensure_device(gfa_plan* p) {
+    std::string err;
+    int dev = 0;
+    hipError_t const e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(GFA_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    if (p->dev.uploaded && p->dev.device != dev)
+        return fail(GFA_ERR_STATE, "the plan was uploaded to another device");
+    if (!p->dev.uploaded) {
+        if (int s = gfa::device_upload_plan(p->host, p->dev, err)) return fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
+    }
+    if (p->values_dirty) {
+        if (int s = gfa::device_upload_values(p->host, p->dev, err)) return fail(GFA_ERR_DEVICE, err);
+        p->values_dirty = false;
+    }
+    return GFA_OK;
+}
+
+gfa::ActionScalars scalars(gfa::Plan const& h) {
+    gfa::ActionScalars s;
+    s.E_re = h.E.real();
+    s.E_im = h.E.imag();
+    std::memcpy(s.phase, h.phase, sizeof(s.phase));
+    for (int d = 0; d < 3; ++d) { s.prefactor[d] = h.kinetic[d].prefactor; s.hgrid[d] = h.hg[d]; }
+    s.hgrid[3] = h.r_proj;
+    return s;
+}
+
+}  // namespace
+
+namespace gfa {
+// used by the solver: run one action on a plan handle
+int capi_action(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, std::string& err) {
+    auto const s = scalars(p->host);
+    if (fp_bits == 64) return device_action<double>(p->dev, s, (double*)y, (double const*)x, stream, err);
+    return device_action<float>(p->dev, s, (float*)y, (float const*)x, stream, err);
+}
+Plan const& capi_host(gfa_plan const* p) { return p->host; }
+DevicePlan& capi_dev(gfa_plan* p) { return p->dev; }
+}  // namespace gfa
+
+extern "C" {
+
+const char* gfa_last_error(void) { return g_err.c_str(); }
+
+int gfa_plan_create(const gfa_plan_desc* desc, gfa_plan** plan) {
+    return guarded([&]() -> int {
+        if (!desc || !plan) return fail(GFA_ERR_ARGUMENT, "null argument");
+        *plan = nullptr;
+        if (desc->natoms < 0 || (desc->natoms > 0 && !desc->xyzZinso)) return fail(GFA_ERR_ARGUMENT, "bad atom list");
+        gfa::PlanInput in;
+        for (int d = 0; d < 3; ++d) {
+            in.ng[d] = desc->ng[d]; in.bc[d] = desc->bc[d]; in.hg[d] = desc->hg[d];
+            in.fd_range[d] = desc->fd_range[d]; in.nsources[d] = desc->nsources[d];
+        }
+        in.xyzZinso.assign(desc->xyzZinso, desc->xyzZinso + 8 * size_t(desc->natoms));
+        in.r_trunc = desc->r_trunc;
+        in.r_proj = desc->r_proj > 0 ? desc->r_proj : 6.0;
+        in.scale_T = desc->scale_T != 0 ? desc->scale_T : 1.0;
+        if (desc->nsources_explicit > 0) {
+            if (!desc->sources) return fail(GFA_ERR_ARGUMENT, "null explicit source list");
+            in.sources.assign(desc->sources, desc->sources + 3 * size_t(desc->nsources_explicit));
+        }
+        auto* p = new gfa_plan();
+        p->echo = desc->echo;
+        std::string err;
+        if (int s = p->host.build(in, err)) {
+            delete p;
+            return fail(GFA_ERR_ARGUMENT, err);
+            (void)s;
+        }
+        if (p->echo > 0)
+            std::printf("# gfa plan: %u rows, %u columns, %llu blocks, %u atoms, %u images\n", p->host.nRows,
+                        p->host.nCols, (unsigned long long)p->host.nnzb, p->host.nAtoms, p->host.nAtomImages);
+        *plan = p;
+        return GFA_OK;
+    });
+}
+
+int gfa_plan_destroy(gfa_plan* plan) {
+    return guarded([&]() -> int {
+        if (!plan) return GFA_OK;
+        gfa::solver_release(plan);
+        if (plan->dev.uploaded || !plan->dev.allocations.empty()) gfa::device_free(plan->dev);
+        delete plan;
+        return GFA_OK;
+    });
+}
+
+int gfa_plan_query(const gfa_plan* plan, gfa_sizes* s) {
+    return guarded([&]() -> int {
+        if (!plan || !s) return fail(GFA_ERR_ARGUMENT, "null argument");
+        auto const& h = plan->host;
+        std::memset(s, 0, sizeof(*s));
+        s->nnzb = h.nnzb; s->nRows = h.nRows; s->nCols = h.nCols;
+        s->nAtoms = h.nAtoms; s->nAtomImages = h.nAtomImages;
+        s->natomcoeffs = h.natomcoeffs(); s->ncoeffs = h.ncoeffs();
+        for (int d = 0; d < 3; ++d) {
+            s->nlists[d] = uint32_t(h.kinetic[d].start.size() - 1);
+            s->fd_range[d] = h.kinetic[d].fd_range;
+            s->nlist_entries[d] = h.kinetic[d].list.size();
+            s->bc[d] = h.bc[d];
+        }
+        s->sho_add_nnz = h.sho_add.index.size();
+        for (auto const& c : h.sho_prj) s->sho_prj_nnz += c.index.size();
+        s->sho_sum_nnz = h.sho_sum.index.size();
+        s->nchunks = plan->dev.nchunks;
+        s->device_bytes = plan->dev.bytes;
+        return GFA_OK;
+    });
+}
+
+int gfa_plan_export(const gfa_plan* plan, int what, void* dst, uint64_t dst_bytes) {
+    return guarded([&]() -> int {
+        if (!plan || !dst) return fail(GFA_ERR_ARGUMENT, "null argument");
+        auto const& h = plan->host;
+        std::vector<char> buf;
+        auto put = [&](void const* src, size_t n) { buf.insert(buf.end(), (char const*)src, (char const*)src + n); };
+        switch (what) {
+            case GFA_EXPORT_ROW_START: put(h.row_start.data(), h.row_start.size() * 4); break;
+            case GFA_EXPORT_COLINDX: put(h.colindx.data(), h.colindx.size() * 2); break;
+            case GFA_EXPORT_SUBSET: put(h.subset.data(), h.subset.size() * 4); break;
+            case GFA_EXPORT_VEFF_INDEX: put(h.veff_index.data(), h.veff_index.size() * 4); break;
+            case GFA_EXPORT_ROW_POS: put(h.row_pos.data(), h.row_pos.size() * 4); break;
+            case GFA_EXPORT_COL_POS: put(h.col_pos.data(), h.col_pos.size() * 4); break;
+            case GFA_EXPORT_SOURCE_IDS: put(h.global_source_ids.data(), h.global_source_ids.size() * 8); break;
+            case GFA_EXPORT_KIN_START_X: case GFA_EXPORT_KIN_START_Y: case GFA_EXPORT_KIN_START_Z: {
+                auto const& k = h.kinetic[what - GFA_EXPORT_KIN_START_X];
+                put(k.start.data(), k.start.size() * 4);
+            } break;
+            case GFA_EXPORT_KIN_LIST_X: case GFA_EXPORT_KIN_LIST_Y: case GFA_EXPORT_KIN_LIST_Z: {
+                auto const& k = h.kinetic[what - GFA_EXPORT_KIN_LIST_X];
+                put(k.list.data(), k.list.size() * 4);
+            } break;
+            case GFA_EXPORT_IMAGE_POS: put(h.image_pos.data(), h.image_pos.size() * 8); break;
+            case GFA_EXPORT_IMAGE_LMAX: put(h.image_lmax.data(), h.image_lmax.size()); break;
+            case GFA_EXPORT_IMAGE_STARTS: put(h.image_starts.data(), h.image_starts.size() * 4); break;
+            case GFA_EXPORT_IMAGE_ATOM: put(h.image_atom.data(), h.image_atom.size() * 4); break;
+            case GFA_EXPORT_IMAGE_SHIFT: put(h.image_shift.data(), h.image_shift.size()); break;
+            case GFA_EXPORT_ATOM_STARTS: put(h.atom_starts.data(), h.atom_starts.size() * 4); break;
+            case GFA_EXPORT_ATOM_LMAX: put(h.atom_lmax.data(), h.atom_lmax.size()); break;
+            case GFA_EXPORT_ATOM_IDS: put(h.global_atom_ids.data(), h.global_atom_ids.size() * 8); break;
+            case GFA_EXPORT_ADD_START: put(h.sho_add.start.data(), h.sho_add.start.size() * 4); break;
+            case GFA_EXPORT_ADD_INDEX: put(h.sho_add.index.data(), h.sho_add.index.size() * 4); break;
+            case GFA_EXPORT_PRJ_START:
+                for (auto const& c : h.sho_prj) put(c.start.data(), c.start.size() * 4);
+                break;
+            case GFA_EXPORT_PRJ_INDEX:
+                for (auto const& c : h.sho_prj) put(c.index.data(), c.index.size() * 4);
+                break;
+            case GFA_EXPORT_SUM_START: put(h.sho_sum.start.data(), h.sho_sum.start.size() * 4); break;
+            case GFA_EXPORT_SUM_INDEX: put(h.sho_sum.index.data(), h.sho_sum.index.size() * 4); break;
+            case GFA_EXPORT_ROWINDX: put(h.rowindx.data(), h.rowindx.size() * 4); break;
+            case GFA_EXPORT_V_CELL: put(h.V_cell.data(), h.V_cell.size() * 8); break;
+            case GFA_EXPORT_ATOM_MATRICES: put(h.atom_matrix.data(), h.atom_matrix.size() * 8); break;
+            case GFA_EXPORT_IMAGE_PHASE: put(h.image_phase.data(), h.image_phase.size() * 8); break;
+            case GFA_EXPORT_PHASE: put(h.phase, sizeof(h.phase)); break;
+            case GFA_EXPORT_KIN_PREFACTOR:
+                for (int d = 0; d < 3; ++d) put(&h.kinetic[d].prefactor, 8);
+                break;
+            default: return fail(GFA_ERR_ARGUMENT, "unknown export id");
+        }
+        if (buf.size() > dst_bytes) return fail(GFA_ERR_ARGUMENT, "destination too small: need " + std::to_string(buf.size()) + " bytes");
+        if (!buf.empty()) std::memcpy(dst, buf.data(), buf.size());
+        return GFA_OK;
+    });
+}
+
+int gfa_set_potential(gfa_plan* plan, const double* V) {
+    return guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        std::string err;
+        if (plan->host.set_potential(V, err)) return fail(GFA_ERR_ARGUMENT, err);
+        plan->values_dirty = true;
+        return GFA_OK;
+    });
+}
+
+int gfa_set_atom_matrices(gfa_plan* plan, int32_t ia, const double* hs, int32_t nc) {
+    return guarded([&]() -> int {
+        if (!plan || !hs) return fail(GFA_ERR_ARGUMENT, "null argument");
+        std::string err;
+        if (plan->host.set_atom_matrices(ia, hs, nc, err)) return fail(GFA_ERR_ARGUMENT, err);
+        plan->host.update_atom_matrices();
+        plan->values_dirty = true;
+        return GFA_OK;
+    });
+}
+
+int gfa_set_energy(gfa_plan* plan, double E_re, double E_im, double dVol) {
+    return guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        if (!(dVol > 0)) return fail(GFA_ERR_ARGUMENT, "dVol must be positive");
+        plan->host.set_energy(std::complex<double>(E_re, E_im), dVol);
+        plan->energy_set = true;
+        plan->values_dirty = true;
+        return GFA_OK;
+    });
+}
+
+int gfa_set_kpoint(gfa_plan* plan, const double k[3]) {
+    return guarded([&]() -> int {
+        if (!plan || !k) return fail(GFA_ERR_ARGUMENT, "null argument");
+        std::string err;
+        if (plan->host.set_kpoint(k, err)) return fail(GFA_ERR_ARGUMENT, err);
+        plan->values_dirty = true;
+        return GFA_OK;
+    });
+}
+
+int gfa_action(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, double* nflops) {
+    return guarded([&]() -> int {
+        if (!plan || !y || !x) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (y == x) return fail(GFA_ERR_ARGUMENT, "the action cannot run in place");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energy before applying the action");
+        if (int s = ensure_device(plan)) return s;
+        std::string err;
+        if (int s = gfa::capi_action(plan, fp_bits, y, x, (hipStream_t)stream, err))
+            return fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
+        if (nflops) *nflops = gfa::reference_flops(plan->host, fp_bits);
+        return GFA_OK;
+    });
+}
+
+int gfa_action_multiply(gfa_plan* plan, int fp_bits, void* y, const void* x, const uint16_t* colIndex,
+                        uint32_t nnzb, uint32_t nCols, uint32_t l2nX, void* stream, int precondition,
+                        double* nflops) {
+    (void)colIndex; (void)l2nX;
+    if (!plan) { g_err = "null plan"; return GFA_ERR_ARGUMENT; }
+    if (nnzb != plan->host.nnzb || nCols != plan->host.nCols) {
+        g_err = "nnzb/nCols do not match the plan";
+        return GFA_ERR_ARGUMENT;
+    }
+    if (precondition) { g_err = "the action has no preconditioner"; return GFA_ERR_ARGUMENT; }
+    return gfa_action(plan, fp_bits, y, x, stream, nflops);
+}
+
+int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve_stats* stats) {
+    return guarded([&]() -> int {
+        if (!plan || !opts) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energy before solving");
+        if (int s = ensure_device(plan)) return s;
+        std::string err;
+        int const s = gfa::solve(plan, *opts, rho, stats, err);
+        if (s) return fail(s, err);
+        return GFA_OK;
+    });
+}
+
+}  // extern "C"

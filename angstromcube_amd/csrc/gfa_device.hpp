@@ -1,0 +1,96 @@
+// Device-side plan and action launchers for gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gfa_plan.hpp"
+
+namespace gfa {
+
+// one projection work item: blocks [begin, end) of the projection list of (atom image, column)
+struct ProjChunk {
+    uint32_t image;
+    uint32_t col;
+    uint32_t begin;
+    uint32_t end;
+};
+
+template <typename real_t>
+struct Workspace {
+    real_t* partial = nullptr;  // [total chunk coefficients][2][64] projection partial sums
+    real_t* dcoef = nullptr;    // [natomcoeffs * nCols][2][64] addition coefficients per atom image
+    size_t partial_elems = 0, dcoef_elems = 0;
+};
+
+struct DevicePlan {
+    bool uploaded = false;
+    int device = -1;
+    uint64_t nnzb = 0;
+    uint32_t nCols = 0, nRows = 0, nAtoms = 0, nAtomImages = 0;
+    int nh_max = 1;                    // neighbour blocks per side in the widest stencil
+    int fd_range[3] = {4, 4, 4};
+
+    // index data
+    uint16_t* colindx = nullptr;       // [nnzb]
+    uint32_t* rowindx = nullptr;       // [nnzb]
+    int32_t* veff = nullptr;           // [nnzb]
+    int32_t* nbr = nullptr;            // [3][2][nh_max][nnzb] neighbour codes
+    float* row_pos = nullptr;          // [nRows][4]
+    uint32_t* add_start = nullptr;     // [nnzb+1]
+    uint32_t* add_index = nullptr;
+    double* image_pos = nullptr;       // [nai][4]
+    int8_t* image_lmax = nullptr;      // [nai]
+    uint32_t* image_starts = nullptr;  // [nai+1]
+    ProjChunk* chunks = nullptr;       // [nchunks]
+    uint32_t* chunk_coef = nullptr;    // [nchunks+1] coefficient offsets of the partial sums
+    uint32_t* prj_blocks = nullptr;    // flat inzb lists of all chunks
+    uint32_t* pair_chunk = nullptr;    // [nai*nCols+1] chunk range of each (image, column)
+    uint32_t* bucket_chunks = nullptr; // chunk ids sorted by numax
+    uint32_t bucket_start[9] = {0};    // ranges into bucket_chunks per numax 0..7
+    uint32_t* sum_start = nullptr;     // [nAtoms+1]
+    uint32_t* sum_index = nullptr;
+    int8_t* atom_lmax = nullptr;       // [nAtoms]
+    uint32_t* atom_starts = nullptr;   // [nAtoms+1]
+    double* cscratch = nullptr;        // [ncoeffs * nCols][2][64] atom coefficients when numax > 5
+    uint64_t ncoeffs = 0;
+    uint32_t max_atom_nc = 0;
+    uint32_t nchunks = 0;
+    uint64_t total_chunk_coefs = 0;
+    uint64_t natomcoeffs = 0;
+
+    // per-system / per-(E,k) data
+    double* V = nullptr;               // [ncell][64]
+    double* atom_mat = nullptr;        // concatenated [2][nc][nc]
+    uint64_t* atom_mat_off = nullptr;  // [nAtoms+1]
+    double* image_phase = nullptr;     // [nai][2]
+
+    Workspace<double> ws64;
+    Workspace<float> ws32;
+    size_t bytes = 0;
+    std::vector<void*> allocations;
+};
+
+struct ActionScalars {
+    double E_re, E_im;
+    double phase[3][2][2];
+    double prefactor[3];
+    double hgrid[4];                  // grid spacings and projection radius (units of sigma)
+};
+
+// Upload the index data (once) and the per-(E,k) data; 0 or error.
+int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err);
+int device_upload_values(Plan const& p, DevicePlan& d, std::string& err);
+void device_free(DevicePlan& d);
+
+template <typename real_t>
+int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
+                  std::string& err);
+
+// flop count of one action as the reference counts it (include/green_action.hxx:154-175)
+double reference_flops(Plan const& p, int fp_bits);
+
+}  // namespace gfa

@@ -1,0 +1,244 @@
+"""ctypes binding of libgfa.so (include/gfa.h) for tests and the benchmark.
+
+The product path is the HIP library; this module only moves pointers.  Device buffers are torch
+tensors on the current HIP device.  Every call raises GfaError with the library's message on a
+non-zero status, and loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgfa.so")
+
+# export ids, in the order of the enum in include/gfa.h
+EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS", "SOURCE_IDS",
+           "KIN_START_X", "KIN_START_Y", "KIN_START_Z", "KIN_LIST_X", "KIN_LIST_Y", "KIN_LIST_Z",
+           "IMAGE_POS", "IMAGE_LMAX", "IMAGE_STARTS", "IMAGE_ATOM", "IMAGE_SHIFT", "ATOM_STARTS",
+           "ATOM_LMAX", "ATOM_IDS", "ADD_START", "ADD_INDEX", "PRJ_START", "PRJ_INDEX", "SUM_START",
+           "SUM_INDEX", "ROWINDX", "V_CELL", "ATOM_MATRICES", "IMAGE_PHASE", "PHASE", "KIN_PREFACTOR"]
+EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
+
+C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply",
+             "gfa_solve", "gfa_last_error"]
+
+
+class GfaError(RuntimeError):
+    pass
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("ng", C.c_uint32 * 3), ("bc", C.c_int8 * 3), ("reserved0", C.c_int8), ("hg", C.c_double * 3),
+                ("xyzZinso", C.POINTER(C.c_double)), ("natoms", C.c_int32), ("echo", C.c_int32),
+                ("r_trunc", C.c_double), ("r_proj", C.c_double), ("fd_range", C.c_int32 * 3),
+                ("nsources", C.c_int32 * 3), ("sources", C.POINTER(C.c_int32)), ("nsources_explicit", C.c_int32),
+                ("reserved1", C.c_int32), ("scale_T", C.c_double)]
+
+
+class Sizes(C.Structure):
+    _fields_ = [("nnzb", C.c_uint64), ("nRows", C.c_uint32), ("nCols", C.c_uint32), ("nAtoms", C.c_uint32),
+                ("nAtomImages", C.c_uint32), ("natomcoeffs", C.c_uint64), ("ncoeffs", C.c_uint64),
+                ("nlists", C.c_uint32 * 3), ("fd_range", C.c_int32 * 3), ("nlist_entries", C.c_uint64 * 3),
+                ("sho_add_nnz", C.c_uint64), ("sho_prj_nnz", C.c_uint64), ("sho_sum_nnz", C.c_uint64),
+                ("bc", C.c_int8 * 3), ("reserved", C.c_int8), ("nchunks", C.c_uint32), ("device_bytes", C.c_uint64)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("fp_bits", C.c_int32), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
+                ("max_refinements", C.c_int32), ("check_every", C.c_int32), ("shadow_seed", C.c_uint64),
+                ("X_device", C.c_void_p), ("stream", C.c_void_p)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged_columns", C.c_int32),
+                ("total_columns", C.c_int32), ("max_residual", C.c_double), ("seconds", C.c_double),
+                ("flops", C.c_double), ("actions", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+
+
+def lib():
+    """Load libgfa.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GfaError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "or make -C angstromcube_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.gfa_plan_create.argtypes = [C.POINTER(PlanDesc), C.POINTER(vp)]
+        L.gfa_plan_destroy.argtypes = [vp]
+        L.gfa_plan_query.argtypes = [vp, C.POINTER(Sizes)]
+        L.gfa_plan_export.argtypes = [vp, C.c_int, vp, C.c_uint64]
+        L.gfa_set_potential.argtypes = [vp, C.POINTER(C.c_double)]
+        L.gfa_set_atom_matrices.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.c_int32]
+        L.gfa_set_energy.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+        L.gfa_set_kpoint.argtypes = [vp, C.POINTER(C.c_double)]
+        L.gfa_action.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_double)]
+        L.gfa_action_multiply.argtypes = [vp, C.c_int, vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int,
+                                          C.POINTER(C.c_double)]
+        L.gfa_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_double), C.POINTER(SolveStats)]
+        L.gfa_last_error.restype = C.c_char_p
+        for name in C_SYMBOLS:
+            getattr(L, name).restype = C.c_int if name != "gfa_last_error" else C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise GfaError(f"gfa status {status}: {lib().gfa_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class GreenPlan:
+    """A Green-function plan: BSR structure, kinetic lists, atom images and the per-(E,k) data.
+
+    Mirrors the reference's call surface: construct_Green_function (constructor), update_potential,
+    update_energy_parameter, update_phases, multiply (action_t::multiply) and solve (green_solver_t::solve).
+    """
+
+    def __init__(self, system, r_trunc, fd_range=(4, 4, 4), nsources=(1, 1, 1), sources=None, r_proj=6.0,
+                 echo=0):
+        L = lib()
+        d = PlanDesc()
+        for k in range(3):
+            d.ng[k] = system.ng[k]
+            d.bc[k] = system.bc[k]
+            d.hg[k] = system.h
+            d.fd_range[k] = fd_range[k]
+            d.nsources[k] = nsources[k]
+        self._xyz = np.ascontiguousarray(system.xyzZinso, dtype=np.float64)
+        d.xyzZinso = _dptr(self._xyz)
+        d.natoms = system.natoms
+        d.echo = echo
+        d.r_trunc = r_trunc
+        d.r_proj = r_proj
+        d.scale_T = 1.0
+        if sources is not None:
+            self._src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32).reshape(-1, 3))
+            d.sources = self._src.ctypes.data_as(C.POINTER(C.c_int32))
+            d.nsources_explicit = self._src.shape[0]
+        h = C.c_void_p()
+        _check(L.gfa_plan_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.system = system
+        self.refresh_sizes()
+
+    def refresh_sizes(self):
+        s = Sizes()
+        _check(lib().gfa_plan_query(self.h, C.byref(s)))
+        self.sizes = s
+        self.nnzb, self.nRows, self.nCols = s.nnzb, s.nRows, s.nCols
+        self.nAtoms, self.nAtomImages = s.nAtoms, s.nAtomImages
+        return s
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gfa_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def export(self, name):
+        s = self.sizes
+        nai, na, nnzb = s.nAtomImages, s.nAtoms, s.nnzb
+        ncell = int(np.prod([n // 4 for n in self.system.ng]))
+        spec = {
+            "ROW_START": (np.uint32, (s.nRows + 1,)), "COLINDX": (np.uint16, (nnzb,)),
+            "SUBSET": (np.uint32, (s.nCols,)), "VEFF_INDEX": (np.int32, (nnzb,)),
+            "ROW_POS": (np.float32, (s.nRows, 4)), "COL_POS": (np.float32, (s.nCols, 4)),
+            "SOURCE_IDS": (np.int64, (s.nCols,)),
+            "IMAGE_POS": (np.float64, (nai, 4)), "IMAGE_LMAX": (np.int8, (nai,)),
+            "IMAGE_STARTS": (np.uint32, (nai + 1,)), "IMAGE_ATOM": (np.uint32, (nai,)),
+            "IMAGE_SHIFT": (np.int8, (nai, 4)), "ATOM_STARTS": (np.uint32, (na + 1,)),
+            "ATOM_LMAX": (np.int8, (na,)), "ATOM_IDS": (np.int64, (na,)),
+            "ADD_START": (np.uint32, (nnzb + 1,)), "ADD_INDEX": (np.uint32, (s.sho_add_nnz,)),
+            "PRJ_START": (np.uint32, (s.nCols, nai + 1)), "PRJ_INDEX": (np.uint32, (s.sho_prj_nnz,)),
+            "SUM_START": (np.uint32, (na + 1,)), "SUM_INDEX": (np.uint32, (s.sho_sum_nnz,)),
+            "ROWINDX": (np.uint32, (nnzb,)), "V_CELL": (np.float64, (ncell, 64)),
+            "IMAGE_PHASE": (np.float64, (nai, 2)), "PHASE": (np.float64, (3, 2, 2)),
+            "KIN_PREFACTOR": (np.float64, (3,)),
+        }
+        for k, d in enumerate("XYZ"):
+            spec["KIN_START_" + d] = (np.uint32, (s.nlists[k] + 1,))
+            spec["KIN_LIST_" + d] = (np.int32, (s.nlist_entries[k],))
+        if name == "ATOM_MATRICES":
+            starts = self.export("ATOM_STARTS")
+            lm = self.export("ATOM_LMAX")
+            n = int(sum(2 * ((l + 1) * (l + 2) * (l + 3) // 6) ** 2 for l in lm))
+            spec[name] = (np.float64, (n,))
+            del starts
+        dt, shape = spec[name]
+        a = np.zeros(shape, dtype=dt)
+        _check(lib().gfa_plan_export(self.h, EXPORT_ID[name], a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return a
+
+    # ---- per-system and per-(E,k) updates ----
+    def update_potential(self, V=None, hs=None):
+        """Potential on the full grid and raw (normalised-basis) atom matrices [2][nc][nc] per atom."""
+        sysm = self.system
+        V = sysm.V if V is None else V
+        hs = sysm.hs if hs is None else hs
+        Vc = np.ascontiguousarray(V, dtype=np.float64)
+        _check(lib().gfa_set_potential(self.h, _dptr(Vc)))
+        for ia, m in enumerate(hs):
+            m = np.ascontiguousarray(m, dtype=np.float64)
+            _check(lib().gfa_set_atom_matrices(self.h, ia, _dptr(m), m.shape[-1]))
+
+    def update_energy_parameter(self, E, dVol=None):
+        dVol = self.system.dvol if dVol is None else dVol
+        _check(lib().gfa_set_energy(self.h, float(E.real), float(E.imag), float(dVol)))
+
+    def update_phases(self, k):
+        kk = (C.c_double * 3)(*k)
+        _check(lib().gfa_set_kpoint(self.h, kk))
+
+    def set_inputs(self, E, k=(0.0, 0.0, 0.0)):
+        self.update_potential()
+        self.update_energy_parameter(E)
+        self.update_phases(k)
+
+    # ---- device work ----
+    def multiply(self, y, x, fp_bits=64, stream=None):
+        """y = (H - E S) x for torch device tensors of shape (nnzb, 2, 64, 64); returns the flop count."""
+        import torch
+        dt = torch.float64 if fp_bits == 64 else torch.float32
+        for t in (x, y):
+            if t.dtype != dt or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (self.nnzb, 2, 64, 64):
+                raise GfaError("x and y must be contiguous device tensors of shape (nnzb, 2, 64, 64)")
+        fl = C.c_double()
+        st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+        _check(lib().gfa_action(self.h, fp_bits, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr()), st, C.byref(fl)))
+        return fl.value
+
+    def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
+              shadow_seed=0, X=None, stream=None):
+        """Solve (H - E S) X = B; returns (rho[nCols, 64] complex, stats dict)."""
+        import torch
+        o = SolveOpts()
+        o.fp_bits = fp_bits
+        o.max_iterations = max_iterations
+        o.tolerance = tolerance
+        o.max_refinements = max_refinements
+        o.check_every = check_every
+        o.shadow_seed = shadow_seed
+        o.X_device = C.c_void_p(X.data_ptr()) if X is not None else None
+        o.stream = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+        rho = np.zeros((self.nCols, 64, 2))
+        st = SolveStats()
+        _check(lib().gfa_solve(self.h, C.byref(o), _dptr(rho), C.byref(st)))
+        self.refresh_sizes()
+        return rho[..., 0] + 1j * rho[..., 1], st.as_dict()

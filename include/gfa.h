@@ -1,0 +1,171 @@
+/*
+ * gfa.h -- C-ABI of the MI355X Green-function action (H - E S) G and its tfQMR solve.
+ *
+ * One plan object replaces the reference's action_plan_t + green_action::action_t + green_solver_t
+ * trio.  Plain pointers and sizes only; device pointers are HIP device addresses.  Every entry point
+ * returns a status (GFA_OK == 0) and never exits the process; gfa_last_error() explains the last
+ * failure of the calling thread.
+ *
+ * Block layout of a Green function X (input and output of the action):
+ *     real_t X[nnzb][2][64][64]      real_t = double (fp_bits 64) or float (fp_bits 32)
+ *     [inzb][re/im plane][target grid point i = (z*4 + y)*4 + x][right-hand side j]
+ * identical to the reference (include/green_action.hxx:141-142).
+ */
+#ifndef GFA_H
+#define GFA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFA_OK            0
+#define GFA_ERR_ARGUMENT  1   /* invalid argument or unsupported option          */
+#define GFA_ERR_DEVICE    2   /* HIP runtime failure (no device, launch failure) */
+#define GFA_ERR_STATE     3   /* call out of order (e.g. no energy set)           */
+#define GFA_ERR_MEMORY    4   /* device or host allocation failed                  */
+
+typedef struct gfa_plan gfa_plan;
+
+/* Boundary conditions: 0 isolated, 1 periodic, 2 vacuum, 3 repeat (include/boundary_condition.hxx:17-40).
+ * A periodic direction whose truncation sphere fits the cell is wrapped internally (code 5). */
+typedef struct {
+    uint32_t ng[3];             /* grid points per direction, multiples of 4                       */
+    int8_t bc[3];               /* boundary condition per direction                                */
+    int8_t reserved0;
+    double hg[3];               /* grid spacing per direction (Bohr)                               */
+    const double* xyzZinso;     /* [natoms*8]: x, y, z, Z, id, numax, sigma, unused (centred frame)*/
+    int32_t natoms;
+    int32_t echo;               /* log level, 0 = silent                                           */
+    double r_trunc;             /* truncation radius (Bohr), green_function.truncation.radius      */
+    double r_proj;              /* projection radius in units of sigma, green_function.projection.radius (6) */
+    int32_t fd_range[3];        /* 0 default (8, or 4 if periodic), 4: 8th order, 8: 16th order   */
+    int32_t nsources[3];        /* box of source blocks at the cell centre (green_function.sources), <0: all */
+    const int32_t* sources;     /* optional explicit source block coordinates [nsources_explicit][3] */
+    int32_t nsources_explicit;
+    int32_t reserved1;
+    double scale_T;             /* hamiltonian.scale.kinetic (1)                                   */
+} gfa_plan_desc;
+
+typedef struct {
+    uint64_t nnzb;              /* non-zero 64x64 blocks of X                                      */
+    uint32_t nRows, nCols;      /* target block rows, source block columns (64 RHS each)           */
+    uint32_t nAtoms, nAtomImages;
+    uint64_t natomcoeffs;       /* SHO coefficients over atom images                               */
+    uint64_t ncoeffs;           /* SHO coefficients over atoms                                     */
+    uint32_t nlists[3];         /* finite-difference lines per direction                           */
+    int32_t fd_range[3];
+    uint64_t nlist_entries[3];
+    uint64_t sho_add_nnz, sho_prj_nnz, sho_sum_nnz;
+    int8_t bc[3];               /* boundary conditions after wrapping (5 = wrap)                   */
+    int8_t reserved;
+    uint32_t nchunks;           /* projection work items                                           */
+    uint64_t device_bytes;      /* device memory currently held by the plan                        */
+} gfa_sizes;
+
+/* arrays for gfa_plan_export (host copies, for verification against the reference) */
+enum {
+    GFA_EXPORT_ROW_START = 0,   /* uint32[nRows+1]    RowStart        src/green_function.cxx:801  */
+    GFA_EXPORT_COLINDX,         /* uint16[nnzb]       colindx                                     */
+    GFA_EXPORT_SUBSET,          /* uint32[nCols]      subset (diagonal blocks)                    */
+    GFA_EXPORT_VEFF_INDEX,      /* int32[nnzb]        veff_index                                  */
+    GFA_EXPORT_ROW_POS,         /* float[nRows][4]    rowCubePos                                  */
+    GFA_EXPORT_COL_POS,         /* float[nCols][4]    colCubePos                                  */
+    GFA_EXPORT_SOURCE_IDS,      /* int64[nCols]       global_source_indices                       */
+    GFA_EXPORT_KIN_START_X,     /* uint32[nlists+1]   finite-difference list starts, x, y, z      */
+    GFA_EXPORT_KIN_START_Y,
+    GFA_EXPORT_KIN_START_Z,
+    GFA_EXPORT_KIN_LIST_X,      /* int32[entries]     finite-difference lists, x, y, z            */
+    GFA_EXPORT_KIN_LIST_Y,
+    GFA_EXPORT_KIN_LIST_Z,
+    GFA_EXPORT_IMAGE_POS,       /* double[nai][4]     AtomImagePos                                */
+    GFA_EXPORT_IMAGE_LMAX,      /* int8[nai]                                                       */
+    GFA_EXPORT_IMAGE_STARTS,    /* uint32[nai+1]                                                   */
+    GFA_EXPORT_IMAGE_ATOM,      /* uint32[nai]        AtomImageIndex                              */
+    GFA_EXPORT_IMAGE_SHIFT,     /* int8[nai][4]                                                    */
+    GFA_EXPORT_ATOM_STARTS,     /* uint32[nAtoms+1]                                                */
+    GFA_EXPORT_ATOM_LMAX,       /* int8[nAtoms]                                                    */
+    GFA_EXPORT_ATOM_IDS,        /* int64[nAtoms]      global_atom_ids                             */
+    GFA_EXPORT_ADD_START,       /* uint32[nnzb+1]     sparse_SHOadd                               */
+    GFA_EXPORT_ADD_INDEX,       /* uint32[nnz]                                                     */
+    GFA_EXPORT_PRJ_START,       /* uint32[nCols][nai+1] sparse_SHOprj, one CSR per column         */
+    GFA_EXPORT_PRJ_INDEX,       /* uint32[nnz] concatenated over columns                          */
+    GFA_EXPORT_SUM_START,       /* uint32[nAtoms+1]   sparse_SHOsum                               */
+    GFA_EXPORT_SUM_INDEX,       /* uint32[nnz]                                                     */
+    GFA_EXPORT_ROWINDX,         /* uint32[nnzb]       rowindx                                     */
+    GFA_EXPORT_V_CELL,          /* double[ncell][64]  cell-indexed local potential               */
+    GFA_EXPORT_ATOM_MATRICES,   /* double[sum 2 nc^2] (h - E s) dVol per contributing atom        */
+    GFA_EXPORT_IMAGE_PHASE,     /* double[nai][2]     Bloch phase of each image                   */
+    GFA_EXPORT_PHASE,           /* double[3][2][2]    kinetic halo phases                         */
+    GFA_EXPORT_KIN_PREFACTOR,   /* double[3]                                                       */
+    GFA_EXPORT_COUNT
+};
+
+/* ---- plan life cycle; replaces green_function::construct_Green_function (include/green_function.hxx:23-31)
+ *      and the action_plan_t constructor/destructor (include/action_plan.hxx:84-95) ---- */
+int gfa_plan_create(const gfa_plan_desc* desc, gfa_plan** plan);
+int gfa_plan_destroy(gfa_plan* plan);
+int gfa_plan_query(const gfa_plan* plan, gfa_sizes* sizes);
+int gfa_plan_export(const gfa_plan* plan, int what, void* dst, uint64_t dst_bytes);
+
+/* ---- per-system and per-(E,k) updates ----
+ * gfa_set_potential replaces green_function::update_potential's potential part (include/green_function.hxx:33-40):
+ *   V[ng_z][ng_y][ng_x] on the full grid, Hartree.
+ * gfa_set_atom_matrices replaces its atom-matrix part plus the SHO prefactor rescaling of
+ *   src/energy_contour.cxx:307-323: hs[2][nc][nc] = h then s of original atom ia in the L2-normalised
+ *   SHO basis (zyx order), nc = nSHO(numax).
+ * gfa_set_energy replaces green_function::update_energy_parameter (include/green_function.hxx:42-49).
+ * gfa_set_kpoint replaces green_function::update_phases (include/green_function.hxx:51-56); k in units of 2 pi. */
+int gfa_set_potential(gfa_plan* plan, const double* V);
+int gfa_set_atom_matrices(gfa_plan* plan, int32_t ia, const double* hs, int32_t nc);
+int gfa_set_energy(gfa_plan* plan, double E_re, double E_im, double dVol);
+int gfa_set_kpoint(gfa_plan* plan, const double k[3]);
+
+/* ---- the action y = (H - E S) x on device pointers; replaces action_t::multiply
+ *      (include/green_action.hxx:140-176).  colIndex may be NULL (the plan holds its own copy); nnzb and
+ *      nCols must match the plan.  stream is a hipStream_t (NULL: default stream).  The call is
+ *      asynchronous on that stream.  *nflops (may be NULL) receives the reference's flop count. ---- */
+int gfa_action(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, double* nflops);
+int gfa_action_multiply(gfa_plan* plan, int fp_bits, void* y, const void* x, const uint16_t* colIndex,
+                        uint32_t nnzb, uint32_t nCols, uint32_t l2nX, void* stream, int precondition,
+                        double* nflops);
+
+/* ---- the solve (H - E S) X = B with B = unit columns on the diagonal blocks; replaces
+ *      green_solver_t::solve / action_t::solve (include/green_solver.hxx:24-30, include/green_action.hxx:181-241)
+ *      and tfqmrgpu::solve(action, buffer, 1e-9, maxiter, 0, true). ---- */
+typedef struct {
+    int32_t fp_bits;            /* 64: plain FP64 tfQMR; 32: FP32 tfQMR inside FP64 refinement        */
+    int32_t max_iterations;     /* tfQMR iterations per (inner) solve                                 */
+    double tolerance;           /* relative residual per column (1e-9)                                */
+    int32_t max_refinements;    /* FP64 outer refinement steps for fp_bits 32 (<= 4 in the workloads) */
+    int32_t check_every;        /* iterations between host convergence checks (0: 8)                  */
+    uint64_t shadow_seed;       /* seed of the random shadow residual                                 */
+    void* X_device;             /* optional: receives the FP64 solution [nnzb][2][64][64] (device)   */
+    void* stream;               /* hipStream_t or NULL                                                */
+} gfa_solve_opts;
+
+typedef struct {
+    int32_t iterations;          /* tfQMR iterations done (sum over refinement steps)                */
+    int32_t refinements;         /* FP64 refinement steps done                                       */
+    int32_t converged_columns;   /* columns whose true FP64 residual met the tolerance               */
+    int32_t total_columns;       /* 64 * nCols                                                       */
+    double max_residual;         /* largest relative residual ||b - A x|| / ||b|| (FP64 action)      */
+    double seconds;              /* wall time of the solve                                           */
+    double flops;                /* action flops (reference counting)                                */
+    int32_t actions;             /* number of action applications                                    */
+    int32_t reserved;
+} gfa_solve_stats;
+
+/* rho[nCols][64][2]: Green function diagonal G_ii / pi (re, im) as include/green_action.hxx:226-238;
+ * may be NULL. */
+int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve_stats* stats);
+
+/* text of the last error of the calling thread ("" if none) */
+const char* gfa_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GFA_H */
