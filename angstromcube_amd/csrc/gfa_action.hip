@@ -17,6 +17,13 @@
 //                     written once.  The block is staged in LDS; neighbour blocks come through L2.
 #include "gfa_device.hpp"
 
+#ifndef LOCAL_LB
+#define LOCAL_LB 2   // kinetic lines per thread loaded together
+#endif
+#ifndef LOCAL_JW
+#define LOCAL_JW 32  // columns per work-group of k_local
+#endif
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -210,15 +217,20 @@ __global__ __launch_bounds__(64) void k_sho_atoms(
 }
 
 // ---------------------------------------------------------------------------------------------
-// 3. local part of the action per block, 256 threads = (j, plane, z half)
+// 3. local part of the action: one work-group per (block, half of the 64 columns), 128 threads =
+//    (column j of the half, plane p, z half hz); each thread produces 32 points of one (j, p).
+//    The block's column half is staged in LDS; neighbour blocks are read with branch-free addressing
+//    (an absent neighbour reads a zero block) so that all loads of a line are in flight together.
 // ---------------------------------------------------------------------------------------------
 template <typename real_t>
 struct LocalArgs {
     real_t* y;
     real_t const* x;
     real_t const* dcoef;
+    real_t const* zero;      // one block of zeros
+    uint32_t const* order;   // processing order of the blocks (cache-aware)
     int32_t const* veff;
-    int32_t const* nbr;
+    int32_t const* nbr;      // [3][2][NH][nnzb]
     double const* V;
     uint32_t const* add_start;
     uint32_t const* add_index;
@@ -230,46 +242,39 @@ struct LocalArgs {
     uint32_t const* image_starts;
     uint64_t nnzb;
     uint32_t nCols;
-    uint32_t nblk_per_xcd;   // XCD-contiguous mapping of work-groups to blocks, 0: identity
+    uint32_t per_xcd;        // work-items per XCD in the XCD-contiguous mapping
     real_t E_re, E_im;
     real_t c8[3][5];         // 8th-order weights per direction
     real_t c16[3][9];        // 16th-order weights per direction
-    int nh[3];               // neighbour blocks per side per direction (1 or 2)
+    int nh[3];               // neighbour blocks per side used in direction d (1 or 2)
     double phase[3][2][2];
     double hgrid[4];
 };
 
-// neighbour value at point i of block code `nb` (>= 0 block, -1 absent, <= -2 periodic image with phase)
-template <typename real_t>
-__device__ __forceinline__ real_t nbr_value(real_t const* __restrict__ x, int32_t nb, int i, int p, int j,
-                                            double const ph[2]) {
-    if (nb >= 0) return x[size_t(nb) * NBLK + p * 4096 + i * 64 + j];
-    if (nb == -1) return real_t(0);
-    size_t const img = size_t(-nb - 2);
-    real_t const v = x[img * NBLK + p * 4096 + i * 64 + j];
-    real_t const w = x[img * NBLK + (1 - p) * 4096 + i * 64 + j];
-    real_t const pr = real_t(ph[0]), pi = real_t(ph[1]) * real_t(1 - 2 * p);
-    return pr * v - pi * w;
-}
-
-// SHO add-back of one atom image for this thread's two z planes
-template <typename real_t, int L>
-__device__ __forceinline__ void sho_add_atom(real_t (&out)[32], real_t const (*H)[3][4], float const (*xi2)[4],
-                                             float R2, real_t const* __restrict__ dsrc, uint32_t nCols, int hz) {
+// SHO add-back of one atom image into the LDS accumulator: this thread's z plane zq of column jl, plane p
+// dsrc points to coefficient 0 of this thread's (plane, column); consecutive coefficients are dstride apart.
+// The loops over SHO indices stay rolled: only the 4x4 spatial loops index register arrays, and rolled
+// loops keep the coefficient reads from being hoisted into registers all at once.
+template <typename real_t, int JW>
+__device__ __forceinline__ void sho_add_atom(real_t* __restrict__ ys, real_t const (*H)[3][4], float const (*xi2)[4],
+                                             float R2, real_t const* __restrict__ dsrc, size_t dstride, int zq, int L) {
     int s = 0;
+    real_t acc[16];
 #pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = 0;
+#pragma unroll 1
     for (int iz = 0; iz <= L; ++iz) {
         real_t byx[4][4];
 #pragma unroll
         for (int y = 0; y < 4; ++y)
 #pragma unroll
             for (int xx = 0; xx < 4; ++xx) byx[y][xx] = 0;
-#pragma unroll
+#pragma unroll 1
         for (int iy = 0; iy <= L - iz; ++iy) {
             real_t ax[4] = {0, 0, 0, 0};
-#pragma unroll
+#pragma unroll 1
             for (int ix = 0; ix <= L - iz - iy; ++ix) {
-                real_t const ca = dsrc[size_t(s) * nCols * 128];
+                real_t const ca = dsrc[size_t(s) * dstride];
                 ++s;
 #pragma unroll
                 for (int xx = 0; xx < 4; ++xx) ax[xx] += H[ix][0][xx] * ca;
@@ -281,160 +286,222 @@ __device__ __forceinline__ void sho_add_atom(real_t (&out)[32], real_t const (*H
                 for (int xx = 0; xx < 4; ++xx) byx[y][xx] += hy_ * ax[xx];
             }
         }
+        real_t const hz_ = H[iz][2][zq];
 #pragma unroll
-        for (int zz = 0; zz < 2; ++zz) {
-            int const z = 2 * hz + zz;
-            real_t const hz_ = H[iz][2][z];
-            float const d2z = xi2[2][z] - R2;
-            if (d2z < 0) {
+        for (int y = 0; y < 4; ++y)
 #pragma unroll
-                for (int y = 0; y < 4; ++y) {
-                    float const d2yz = xi2[1][y] + d2z;
-                    if (d2yz < 0) {
+            for (int xx = 0; xx < 4; ++xx) acc[y * 4 + xx] += hz_ * byx[y][xx];
+    }
+    float const d2z = xi2[2][zq] - R2;
 #pragma unroll
-                        for (int xx = 0; xx < 4; ++xx) {
-                            float const d2 = xi2[0][xx] + d2yz;
-                            if (d2 < 0) out[(zz * 4 + y) * 4 + xx] += hz_ * byx[y][xx];
-                        }
-                    }
-                }
-            }
+    for (int y = 0; y < 4; ++y) {
+        float const d2yz = xi2[1][y] + d2z;
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx) {
+            float const d2 = xi2[0][xx] + d2yz;
+            bool const in = (d2z < 0) && (d2yz < 0) && (d2 < 0);
+            real_t& o = ys[(zq * 16 + y * 4 + xx) * JW];
+            o += in ? acc[y * 4 + xx] : real_t(0);
         }
     }
 }
 
-template <typename real_t, int NHMAX>
-__global__ __launch_bounds__(256) void k_local(LocalArgs<real_t> a) {
-    __shared__ real_t xs[NBLK];
+// One work-group per (block, JW-column slice), 8*JW threads.  LDS holds the slice of the block
+// (xs) and the output accumulator (ys); passes: potential, kinetic x, y, z, SHO add-back, store.
+// In the kinetic passes a thread owns 4 lines (4 points each) of one (column, plane); neighbour
+// values come straight from global memory with branch-free addressing (absent -> zero block).
+template <typename real_t, int NH, bool IMG, int JW>
+__global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
+    constexpr int T = 8 * JW;
+    constexpr int NW = 4 * (2 * NH + 1);   // points in a line window
+    constexpr int NSLICE = 64 / JW;
+    constexpr int LB = LOCAL_LB;
+    __shared__ real_t xs[2 * 64 * JW];
+    __shared__ real_t ys[2 * 64 * JW];
     __shared__ real_t H[8][3][4];
     __shared__ float xi2[3][4];
     __shared__ double apos[4];
     __shared__ float cpos[4];
     __shared__ double hgrid[4];
 
-    uint64_t blk = blockIdx.x;
-    if (a.nblk_per_xcd) blk = uint64_t(blockIdx.x & 7) * a.nblk_per_xcd + (blockIdx.x >> 3);
-    if (blk >= a.nnzb) return;  // uniform over the work-group
-    int const tid = threadIdx.x, j = tid & 63, p = (tid >> 6) & 1, hz = tid >> 7;
+    uint64_t lin = blockIdx.x;
+    if (a.per_xcd) lin = uint64_t(blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
+    uint64_t const item = lin / NSLICE;
+    int const slice = int(lin % NSLICE);
+    if (item >= a.nnzb) return;  // uniform over the work-group
+    uint64_t const blk = a.order[item];
+    int const tid = threadIdx.x;
 
-    // stage the block in LDS with 16-byte loads
+    // stage this column slice of the block in LDS with 16-byte loads
+    using vec_t = typename std::conditional<sizeof(real_t) == 8, double2, float4>::type;
+    constexpr int VE = 16 / sizeof(real_t), VPR = JW / VE;  // vectors per row
     {
-        using vec_t = typename std::conditional<sizeof(real_t) == 8, double2, float4>::type;
-        constexpr int VE = 16 / sizeof(real_t);
-        vec_t const* src = reinterpret_cast<vec_t const*>(a.x + blk * NBLK);
-        vec_t* dst = reinterpret_cast<vec_t*>(xs);
+        real_t const* src = a.x + blk * NBLK + slice * JW;
 #pragma unroll
-        for (int k = 0; k < NBLK / VE / 256; ++k) dst[k * 256 + tid] = src[k * 256 + tid];
+        for (int k = 0; k < 2 * 64 * VPR / T; ++k) {
+            int const v = k * T + tid, row = v / VPR, cv = v % VPR;
+            *reinterpret_cast<vec_t*>(xs + row * JW + cv * VE) = *reinterpret_cast<vec_t const*>(src + row * 64 + cv * VE);
+        }
     }
     if (tid < 4) hgrid[tid] = a.hgrid[tid];
     __syncthreads();
 
-    real_t out[32];
-    real_t const* xsp = xs + p * 4096 + j;
-    // ---- local potential: (V - E) x ----
+    // ---- potential: ys = (V - E) x ----
     {
         int32_t const iv = a.veff[blk];
-        real_t const Vim = a.E_im * real_t(1 - 2 * p);
-        real_t const* xso = xs + (1 - p) * 4096 + j;
+        real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
+        double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64;
 #pragma unroll
-        for (int il = 0; il < 32; ++il) {
-            int const i = hz * 32 + il;
-            real_t const Vd = (iv < 0) ? real_t(0) : real_t(a.V[size_t(iv) * 64 + i]);
-            out[il] = (Vd - a.E_re) * xsp[i * 64] + Vim * xso[i * 64];
+        for (int k = 0; k < 2 * 64 * JW / T; ++k) {
+            int const e = k * T + tid, jl = e % JW, row = e / JW, p = row >> 6, i = row & 63;
+            real_t const Vim = a.E_im * real_t(1 - 2 * p);
+            ys[e] = (real_t(Vb[i]) * vm - a.E_re) * xs[e] + Vim * xs[((1 - p) * 64 + i) * JW + jl];
         }
     }
 
-    // ---- kinetic energy, one direction at a time ----
+    // thread's (column, plane, line group) in the kinetic passes
+    int const jl = tid % JW, p = (tid / JW) & 1, lg = tid / (2 * JW);   // lg in 0..3
+    int const j = slice * JW + jl;
+    real_t* ysp = ys + p * 64 * JW + jl;
+    real_t const* xsp = xs + p * 64 * JW + jl;
+    size_t const off_p = size_t(p) * 4096 + j, off_o = size_t(1 - p) * 4096 + j;
+
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         int const stride = (d == 0) ? 1 : ((d == 1) ? 4 : 16);
-        int32_t nbl[2], nbr_[2];
+        // neighbour blocks of this direction (uniform over the work-group)
+        real_t const* nb[2][NH];
+        real_t fr[2][NH], fi[2][NH];
 #pragma unroll
-        for (int k = 0; k < NHMAX; ++k) {
-            nbl[k] = a.nbr[((d * 2 + 0) * NHMAX + k) * a.nnzb + blk];
-            nbr_[k] = a.nbr[((d * 2 + 1) * NHMAX + k) * a.nnzb + blk];
-        }
-        bool const wide = (NHMAX == 2) && (a.nh[d] == 2);
-        // 16 lines of 4 points in direction d; lines are indexed by the two other coordinates
+        for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
-        for (int line = 0; line < 16; ++line) {
-            int i0;  // first point of the line
-            if (d == 0) i0 = line * 4;                                  // line = z*4 + y
-            else if (d == 1) i0 = (line >> 2) * 16 + (line & 3);        // line = z*4 + x
-            else i0 = line;                                             // line = y*4 + x
-            // which of this line's points belong to this thread
-            int zsel;  // for d < 2: line belongs to the thread if its z is in the thread's half
-            if (d < 2) { zsel = line >> 2; if ((zsel >> 1) != hz) continue; }
-            real_t w[20];  // positions -8..11 relative to the block start along the line
+            for (int k = 0; k < NH; ++k) {
+                int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+                uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
+                nb[sd][k] = (code == -1) ? a.zero : a.x + idx * NBLK;
+                bool const img = IMG && (code <= -2);
+                fr[sd][k] = img ? real_t(a.phase[d][sd][0]) : real_t(1);
+                fi[sd][k] = img ? real_t(a.phase[d][sd][1]) * real_t(1 - 2 * p) : real_t(0);
+            }
+        __syncthreads();  // the previous pass has finished its updates of ys
+#pragma unroll 1
+        for (int rb = 0; rb < 4; rb += LB) {
+        real_t w[LB][NW];
+        int il0[LB];
 #pragma unroll
-            for (int q = 0; q < 20; ++q) w[q] = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) w[8 + q] = xsp[(i0 + q * stride) * 64];
-            // for the z direction the 9-point stencil of this thread's two planes needs only part
-            // of the neighbour planes
-            int const qlo = (d == 2 && !wide) ? 2 * hz : 0;       // left neighbour planes qlo..3
-            int const qhi = (d == 2 && !wide) ? 2 * hz + 1 : 3;   // right neighbour planes 0..qhi
+        for (int r = 0; r < LB; ++r) {
+            int const line = lg * 4 + rb + r;  // 0..15
+            int i0;
+            if (d == 0) i0 = line * 4;                               // line = z*4 + y
+            else if (d == 1) i0 = (line >> 2) * 16 + (line & 3);     // line = z*4 + x
+            else i0 = line;                                          // line = y*4 + x
+            il0[r] = i0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (q >= qlo) w[4 + q] = nbr_value(a.x, nbl[0], i0 + q * stride, p, j, a.phase[d][0]);
-                if (q <= qhi) w[12 + q] = nbr_value(a.x, nbr_[0], i0 + q * stride, p, j, a.phase[d][1]);
-                if (wide) {
-                    if (d < 2 || q >= 2 * hz) w[q] = nbr_value(a.x, nbl[NHMAX - 1], i0 + q * stride, p, j, a.phase[d][0]);
-                    if (d < 2 || q <= 2 * hz + 1) w[16 + q] = nbr_value(a.x, nbr_[NHMAX - 1], i0 + q * stride, p, j, a.phase[d][1]);
+                int const i = i0 + q * stride;
+                w[r][4 * NH + q] = xsp[i * JW];
+#pragma unroll
+                for (int k = 0; k < NH; ++k) {
+                    real_t vl = nb[0][k][off_p + i * 64], vr = nb[1][k][off_p + i * 64];
+                    if (IMG) {
+                        vl = fr[0][k] * vl - fi[0][k] * nb[0][k][off_o + i * 64];
+                        vr = fr[1][k] * vr - fi[1][k] * nb[1][k][off_o + i * 64];
+                    }
+                    w[r][4 * (NH - 1 - k) + q] = vl;
+                    w[r][4 * (NH + 1 + k) + q] = vr;
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int const i = i0 + k * stride;            // point in the block
-                int const il = i - hz * 32;               // local output index
-                if (d == 2 && (il < 0 || il >= 32)) continue;
-                int const c = 8 + k;
-                real_t t;
-                if (wide) {
-                    t = a.c16[d][0] * w[c];
-#pragma unroll
-                    for (int m = 1; m <= 8; ++m) t += a.c16[d][m] * (w[c - m] + w[c + m]);
-                } else {
-                    t = a.c8[d][0] * w[c];
-#pragma unroll
-                    for (int m = 1; m <= 4; ++m) t += a.c8[d][m] * (w[c - m] + w[c + m]);
-                }
-                out[il] += t;
             }
         }
+#pragma unroll
+        for (int r = 0; r < LB; ++r) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int const c = 4 * NH + q;
+                real_t t;
+                if (NH == 2 && a.nh[d] == 2) {
+                    t = a.c16[d][0] * w[r][c];
+#pragma unroll
+                    for (int m = 1; m <= 8; ++m) {
+                        int const lo = c - m, hi = c + m;
+                        t += a.c16[d][m] * (w[r][lo < 0 ? 0 : lo] + w[r][hi >= NW ? NW - 1 : hi]);
+                    }
+                } else {
+                    t = a.c8[d][0] * w[r][c];
+#pragma unroll
+                    for (int m = 1; m <= 4; ++m) t += a.c8[d][m] * (w[r][c - m] + w[r][c + m]);
+                }
+                ysp[(il0[r] + q * stride) * JW] += t;
+            }
+        }
+        }  // line batches
     }
 
-    // ---- SHO add-back of all atom images touching this block ----
+    // ---- SHO add-back of the atom images (numax <= 5) touching this block; the coefficients of
+    //      each image are staged in the LDS buffer of the block, which is free after the kinetic passes ----
     uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
     if (b1 > b0) {
         uint32_t const col = a.colindx[blk];
         float const R2 = float(a.hgrid[3] * a.hgrid[3]);
+        int const zq = lg;  // threads (jl, p, zq) own the z plane zq
+        real_t* cs = xs;    // [nc][2][JW]
         for (uint32_t b = b0; b < b1; ++b) {
             uint32_t const img = a.add_index[b];
             int const L = a.image_lmax[img];
+            if (L > 5) continue;  // done by k_sho_add_wide
+            int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
             __syncthreads();
             if (tid < 4) apos[tid] = a.image_pos[4 * size_t(img) + tid];
             if (tid < 3) cpos[tid] = a.row_pos[4 * size_t(a.rowindx[blk]) + tid];
+            {
+                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
+                for (int v = tid; v < 2 * nc * VPR; v += T) {
+                    int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
+                    *reinterpret_cast<vec_t*>(cs + row * JW + cv * VE) =
+                        *reinterpret_cast<vec_t const*>(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * VE);
+                }
+            }
             __syncthreads();
             if (tid < 12) hermite_1d<real_t>(H, xi2, tid, L, apos, cpos, hgrid);
             __syncthreads();
-            real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + p * 64 + j;
-            switch (L) {
-                case 0: sho_add_atom<real_t, 0>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-                case 1: sho_add_atom<real_t, 1>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-                case 2: sho_add_atom<real_t, 2>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-                case 3: sho_add_atom<real_t, 3>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-                case 4: sho_add_atom<real_t, 4>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-                case 5: sho_add_atom<real_t, 5>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-                case 6: sho_add_atom<real_t, 6>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-                default: sho_add_atom<real_t, 7>(out, H, xi2, R2, dsrc, a.nCols, hz); break;
-            }
+            sho_add_atom<real_t, JW>(ysp, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
         }
     }
+    __syncthreads();
 
-    real_t* yb = a.y + blk * NBLK + p * 4096 + j;
+    // ---- store the slice with 16-byte stores ----
+    {
+        real_t* dst = a.y + blk * NBLK + slice * JW;
 #pragma unroll
-    for (int il = 0; il < 32; ++il) yb[(hz * 32 + il) * 64] = out[il];
+        for (int k = 0; k < 2 * 64 * VPR / T; ++k) {
+            int const v = k * T + tid, row = v / VPR, cv = v % VPR;
+            *reinterpret_cast<vec_t*>(dst + row * 64 + cv * VE) = *reinterpret_cast<vec_t const*>(ys + row * JW + cv * VE);
+        }
+    }
+}
+
+
+// SHO add-back of atom images with numax 6 or 7, y += P d, for the (block, image) pairs listed in
+// pairs[]: one work-group of 256 threads = (column j, plane p, z plane zq) per pair.  Runs after k_local.
+template <typename real_t>
+__global__ __launch_bounds__(512) void k_sho_add_wide(LocalArgs<real_t> a, uint32_t const* __restrict__ pairs) {
+    __shared__ real_t H[8][3][4];
+    __shared__ float xi2[3][4];
+    __shared__ double apos[4];
+    __shared__ float cpos[4];
+    __shared__ double hgrid[4];
+    uint32_t const blk = pairs[2 * blockIdx.x], img = pairs[2 * blockIdx.x + 1];
+    int const tid = threadIdx.x, j = tid & 63, p = (tid >> 6) & 1, zq = tid >> 7;
+    int const L = a.image_lmax[img];
+    if (tid < 4) { apos[tid] = a.image_pos[4 * size_t(img) + tid]; hgrid[tid] = a.hgrid[tid]; }
+    if (tid < 3) cpos[tid] = a.row_pos[4 * size_t(a.rowindx[blk]) + tid];
+    __syncthreads();
+    if (tid < 12) hermite_1d<real_t>(H, xi2, tid, L, apos, cpos, hgrid);
+    __syncthreads();
+    float const R2 = float(a.hgrid[3] * a.hgrid[3]);
+    real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + a.colindx[blk]) * 128 + p * 64 + j;
+    real_t* yb = a.y + size_t(blk) * NBLK + p * 4096 + j;
+    // ys stride 64 for the direct update of y
+    sho_add_atom<real_t, 64>(yb, H, xi2, R2, dsrc, size_t(a.nCols) * 128, zq, L);
 }
 
 template <typename T>
@@ -506,6 +573,26 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
     if ((s = dupload(d, d.rowindx, p.rowindx, err))) return s;
     if ((s = dupload(d, d.veff, p.veff_index, err))) return s;
     if ((s = dupload(d, d.row_pos, p.row_pos, err))) return s;
+    {   // processing order of k_local: per source column, xy tiles of TILE x TILE target blocks swept
+        // along z, so that the 6 stencil neighbours of a block were read shortly before (z neighbours
+        // TILE^2 blocks back); consecutive work items then share one XCD's L2
+        int const TILE = 4;
+        std::vector<uint64_t> key(p.nnzb);
+        int32_t lo[3] = {1 << 30, 1 << 30, 1 << 30};
+        for (uint32_t r = 0; r < p.nRows; ++r)
+            for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], p.row_coords[3 * r + k]);
+        for (uint64_t b = 0; b < p.nnzb; ++b) {
+            int32_t const* g = &p.row_coords[3 * size_t(p.rowindx[b])];
+            uint64_t const x = uint64_t(g[0] - lo[0]), y = uint64_t(g[1] - lo[1]), z = uint64_t(g[2] - lo[2]);
+            uint64_t const tx = x / TILE, ty = y / TILE;
+            uint64_t const txs = (ty & 1) ? (1023 - tx) : tx;  // serpentine over tiles
+            key[b] = (uint64_t(p.colindx[b]) << 48) | (ty << 38) | (txs << 28) | (z << 8) | ((y % TILE) << 4) | (x % TILE);
+        }
+        std::vector<uint32_t> order(p.nnzb);
+        for (uint64_t b = 0; b < p.nnzb; ++b) order[b] = uint32_t(b);
+        std::sort(order.begin(), order.end(), [&](uint32_t u, uint32_t v) { return key[u] < key[v]; });
+        if ((s = dupload(d, d.order, order, err))) return s;
+    }
 
     // neighbour codes from the finite-difference lists: the entry before (after) a block in its
     // list is its left (right) neighbour; halo entries 0 are absent, negative ones periodic images
@@ -529,6 +616,7 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
                             else if (e == 0) code = -1;
                             else code = -2 - (-e - CUBE_EXISTS);
                             nb[((size_t(dd) * 2 + side) * NH + k) * p.nnzb + inzb] = code;
+                            if (code <= -2) d.has_images = true;
                         }
                     }
                 }
@@ -540,6 +628,14 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
     // dyadic structures
     if ((s = dupload(d, d.add_start, p.sho_add.start, err))) return s;
     if ((s = dupload(d, d.add_index, p.sho_add.index, err))) return s;
+    {   // (block, image) pairs with numax > 5, added by a separate kernel after k_local
+        std::vector<uint32_t> wide;
+        for (uint64_t blk = 0; blk < p.nnzb; ++blk)
+            for (uint32_t k = p.sho_add.start[blk]; k < p.sho_add.start[blk + 1]; ++k)
+                if (p.image_lmax[p.sho_add.index[k]] > 5) { wide.push_back(uint32_t(blk)); wide.push_back(p.sho_add.index[k]); }
+        d.n_wide_pairs = uint32_t(wide.size() / 2);
+        if ((s = dupload(d, d.wide_pairs, wide, err))) return s;
+    }
     if ((s = dupload(d, d.image_pos, p.image_pos, err))) return s;
     if ((s = dupload(d, d.image_lmax, p.image_lmax, err))) return s;
     if ((s = dupload(d, d.image_starts, p.image_starts, err))) return s;
@@ -653,6 +749,12 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     a.add_start = d.add_start; a.add_index = d.add_index; a.colindx = d.colindx; a.rowindx = d.rowindx;
     a.row_pos = d.row_pos; a.image_pos = d.image_pos; a.image_lmax = d.image_lmax; a.image_starts = d.image_starts;
     a.nnzb = d.nnzb; a.nCols = d.nCols;
+    if (!ws.zero) {
+        if (int r = dalloc(d, ws.zero, NBLK, err)) return r;
+        HIP_TRY(hipMemsetAsync(ws.zero, 0, NBLK * sizeof(real_t), stream));
+    }
+    a.zero = ws.zero;
+    a.order = d.order;
     a.E_re = real_t(s.E_re); a.E_im = real_t(s.E_im);
     static int const c8i[5] = {-14350, 8064, -1008, 128, -9};
     static int const c16i[9] = {-924708642, 538137600, -94174080, 22830080, -5350800, 1053696, -156800, 15360, -735};
@@ -664,14 +766,27 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
         for (int u = 0; u < 2; ++u) for (int v = 0; v < 2; ++v) a.phase[dd][u][v] = s.phase[dd][u][v];
     }
     for (int k = 0; k < 4; ++k) a.hgrid[k] = s.hgrid[k];
-    // work-groups are dealt round-robin to the 8 XCDs: give each XCD a contiguous range of blocks
-    uint64_t const per = (d.nnzb + 7) / 8;
-    a.nblk_per_xcd = uint32_t(per);
+    // work-groups are dealt round-robin to the 8 XCDs: give each XCD a contiguous range of
+    // (block, column half) items so that neighbour blocks meet in the same L2
+    constexpr int JW = LOCAL_JW;
+    uint64_t const items = (64 / JW) * d.nnzb;
+    uint64_t const per = (items + 7) / 8;
+    a.per_xcd = uint32_t(per);
     uint64_t const grid = per * 8;
     if (grid > 0x7fffffffull) { err = "too many blocks for one launch"; return 1; }
-    if (d.nh_max == 2) hipLaunchKernelGGL((k_local<real_t, 2>), dim3(uint32_t(grid)), dim3(256), 0, stream, a);
-    else hipLaunchKernelGGL((k_local<real_t, 1>), dim3(uint32_t(grid)), dim3(256), 0, stream, a);
+    dim3 const g{uint32_t(grid), 1, 1}, b{8 * JW, 1, 1};
+    if (d.nh_max == 2) {
+        if (d.has_images) hipLaunchKernelGGL((k_local<real_t, 2, true, JW>), g, b, 0, stream, a);
+        else hipLaunchKernelGGL((k_local<real_t, 2, false, JW>), g, b, 0, stream, a);
+    } else {
+        if (d.has_images) hipLaunchKernelGGL((k_local<real_t, 1, true, JW>), g, b, 0, stream, a);
+        else hipLaunchKernelGGL((k_local<real_t, 1, false, JW>), g, b, 0, stream, a);
+    }
     HIP_TRY(hipGetLastError());
+    if (d.n_wide_pairs > 0) {
+        hipLaunchKernelGGL((k_sho_add_wide<real_t>), dim3(d.n_wide_pairs), dim3(512), 0, stream, a, d.wide_pairs);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
 }
 

@@ -105,10 +105,9 @@ int gfa_plan_create(const gfa_plan_desc* desc, gfa_plan** plan) {
         auto* p = new gfa_plan();
         p->echo = desc->echo;
         std::string err;
-        if (int s = p->host.build(in, err)) {
+        if (p->host.build(in, err) != 0) {
             delete p;
             return fail(GFA_ERR_ARGUMENT, err);
-            (void)s;
         }
         if (p->echo > 0)
             std::printf("# gfa plan: %u rows, %u columns, %llu blocks, %u atoms, %u images\n", p->host.nRows,

@@ -23,6 +23,7 @@ template <typename real_t>
 struct Workspace {
     real_t* partial = nullptr;  // [total chunk coefficients][2][64] projection partial sums
     real_t* dcoef = nullptr;    // [natomcoeffs * nCols][2][64] addition coefficients per atom image
+    real_t* zero = nullptr;     // one block of zeros, read in place of absent neighbours
     size_t partial_elems = 0, dcoef_elems = 0;
 };
 
@@ -32,6 +33,7 @@ struct DevicePlan {
     uint64_t nnzb = 0;
     uint32_t nCols = 0, nRows = 0, nAtoms = 0, nAtomImages = 0;
     int nh_max = 1;                    // neighbour blocks per side in the widest stencil
+    bool has_images = false;           // some finite-difference halo is a periodic image
     int fd_range[3] = {4, 4, 4};
 
     // index data
@@ -39,9 +41,12 @@ struct DevicePlan {
     uint32_t* rowindx = nullptr;       // [nnzb]
     int32_t* veff = nullptr;           // [nnzb]
     int32_t* nbr = nullptr;            // [3][2][nh_max][nnzb] neighbour codes
+    uint32_t* order = nullptr;         // [nnzb] processing order of k_local
     float* row_pos = nullptr;          // [nRows][4]
     uint32_t* add_start = nullptr;     // [nnzb+1]
     uint32_t* add_index = nullptr;
+    uint32_t* wide_pairs = nullptr;    // (block, image) pairs with numax > 5
+    uint32_t n_wide_pairs = 0;
     double* image_pos = nullptr;       // [nai][4]
     int8_t* image_lmax = nullptr;      // [nai]
     uint32_t* image_starts = nullptr;  // [nai+1]

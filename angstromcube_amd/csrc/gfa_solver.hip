@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -476,10 +477,11 @@ void** vecs(SolverWs& ws);
 template <> void** vecs<double>(SolverWs& ws) { return ws.v64; }
 template <> void** vecs<float>(SolverWs& ws) { return ws.v32; }
 
+// allocate vectors lo..hi-1 of one precision (each nnzb blocks)
 template <typename real_t>
-int ensure_vectors(SolverWs& ws, int n, std::string& err) {
+int ensure_vectors(SolverWs& ws, int lo, int hi, std::string& err) {
     void** v = vecs<real_t>(ws);
-    for (int k = 0; k < n; ++k) {
+    for (int k = lo; k < hi; ++k) {
         if (v[k]) continue;
         real_t* p = nullptr;
         if (int s = walloc(ws, p, size_t(ws.nnzb) * NBLK, err)) return s;
@@ -502,7 +504,7 @@ template <typename real_t>
 int tfqmr(gfa_plan* plan, SolverState& st, real_t* x, real_t const* rhs, double tol_rel, double const* tol_abs,
           int maxit, int check_every, uint64_t seed, hipStream_t stream, TfqmrResult& res, std::string& err) {
     auto& ws = st.ws;
-    if (int s = ensure_vectors<real_t>(ws, 8, err)) return s;
+    if (int s = ensure_vectors<real_t>(ws, 1, 8, err)) return s;
     void** V = vecs<real_t>(ws);
     // V[0] x is the caller's x; buffers: w, u0, u1, v, d, Aodd, Aeven
     real_t* w = (real_t*)V[1];
@@ -603,7 +605,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     gfa_solve_stats S;
     std::memset(&S, 0, sizeof(S));
     S.total_columns = int32_t(ws.ncol);
-    if (int s = ensure_vectors<double>(ws, 10, err)) return s;
+    if (int s = ensure_vectors<double>(ws, 8, 10, err)) return s;
     double* x64 = (double*)ws.v64[8];   // the FP64 solution
     double* y64 = (double*)ws.v64[9];   // FP64 action output
     std::vector<double> resid(ws.ncol);
@@ -619,7 +621,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     }
 
     if (o.fp_bits == 64) {
-        if (int s = ensure_vectors<double>(ws, 11, err)) return s;
+        if (int s = ensure_vectors<double>(ws, 0, 11, err)) return s;
         double* x = (double*)ws.v64[0];
         double* rhs64 = (double*)ws.v64[10];
         TfqmrResult r;
@@ -628,6 +630,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         HIP_TRY(hipMemcpyAsync(x64, x, ws.nnzb * NBLK * sizeof(double), hipMemcpyDeviceToDevice, stream));
         if (int s = residual<double>(plan, st, x64, y64, rhs64, stream, S.actions, err)) return s;
         if (int s = fetch_resid()) return s;
+        if (o.echo > 0) std::fprintf(stderr, "# gfa_solve fp64: %d half-steps, max residual %.3e\n", r.halfsteps, max_resid());
         // FP64 refinement steps if requested and needed: solve A delta = r, x += delta
         for (int k = 0; k < o.max_refinements && max_resid() > o.tolerance; ++k) {
             TfqmrResult r2;
@@ -638,11 +641,11 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             if (int s = fetch_resid()) return s;
         }
     } else {
-        if (int s = ensure_vectors<float>(ws, 9, err)) return s;
+        if (int s = ensure_vectors<float>(ws, 0, 9, err)) return s;
         float* x32 = (float*)ws.v32[0];
         float* rhs32 = (float*)ws.v32[8];
         HIP_TRY(hipMemsetAsync(x64, 0, ws.nnzb * NBLK * sizeof(double), stream));
-        double const floor_rel = 1e-5;  // what one FP32 solve is asked to gain
+        double const floor_rel = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-5;  // gain asked of one FP32 solve
         for (int k = 0; k <= o.max_refinements; ++k) {
             TfqmrResult r;
             if (int s = tfqmr<float>(plan, st, x32, k == 0 ? nullptr : rhs32, floor_rel, ws.tol_abs, o.max_iterations,
@@ -652,6 +655,8 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             hipLaunchKernelGGL((k_accumulate<float>), dim3(uint32_t(ws.nnzb)), dim3(256), 0, stream, x64, x32);
             if (int s = residual<float>(plan, st, x64, y64, rhs32, stream, S.actions, err)) return s;
             if (int s = fetch_resid()) return s;
+            if (o.echo > 0)
+                std::fprintf(stderr, "# gfa_solve fp32 step %d: %d half-steps, max FP64 residual %.3e\n", k, r.halfsteps, max_resid());
             if (max_resid() <= o.tolerance) break;
         }
     }

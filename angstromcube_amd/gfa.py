@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgfa.so")
+# GFA_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GFA_LIBRARY") or os.path.join(HERE, "libgfa.so")
 
 # export ids, in the order of the enum in include/gfa.h
 EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS", "SOURCE_IDS",
@@ -50,7 +51,8 @@ class Sizes(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("fp_bits", C.c_int32), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
                 ("max_refinements", C.c_int32), ("check_every", C.c_int32), ("shadow_seed", C.c_uint64),
-                ("X_device", C.c_void_p), ("stream", C.c_void_p)]
+                ("X_device", C.c_void_p), ("stream", C.c_void_p), ("inner_tolerance", C.c_double),
+                ("echo", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -150,7 +152,10 @@ class GreenPlan:
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: module globals may already be gone
+            pass
 
     def export(self, name):
         s = self.sizes
@@ -178,7 +183,7 @@ class GreenPlan:
         if name == "ATOM_MATRICES":
             starts = self.export("ATOM_STARTS")
             lm = self.export("ATOM_LMAX")
-            n = int(sum(2 * ((l + 1) * (l + 2) * (l + 3) // 6) ** 2 for l in lm))
+            n = int(sum(2 * ((l + 1) * (l + 2) * (l + 3) // 6) ** 2 for l in lm.astype(int)))
             spec[name] = (np.float64, (n,))
             del starts
         dt, shape = spec[name]
@@ -225,7 +230,7 @@ class GreenPlan:
         return fl.value
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
-              shadow_seed=0, X=None, stream=None):
+              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0):
         """Solve (H - E S) X = B; returns (rho[nCols, 64] complex, stats dict)."""
         import torch
         o = SolveOpts()
@@ -235,6 +240,8 @@ class GreenPlan:
         o.max_refinements = max_refinements
         o.check_every = check_every
         o.shadow_seed = shadow_seed
+        o.inner_tolerance = inner_tolerance
+        o.echo = echo
         o.X_device = C.c_void_p(X.data_ptr()) if X is not None else None
         o.stream = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         rho = np.zeros((self.nCols, 64, 2))

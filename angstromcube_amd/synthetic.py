@@ -44,6 +44,18 @@ class SplitMix64:
         return sigma * math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
 
 
+def hash_uniform(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """Counter-based deviates in [-1, 1): the SplitMix64 output function applied to seed + (i+1)*gamma,
+    top 53 bits.  Vectorised; used for seeded test vectors (identical on every platform)."""
+    i = np.arange(offset, offset + n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (i + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+
+
 def nsho(numax: int) -> int:
     """Number of 3D SHO functions up to numax (include/sho_tools.hxx:21-27)."""
     return (numax + 1) * (numax + 2) * (numax + 3) // 6

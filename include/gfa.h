@@ -143,6 +143,9 @@ typedef struct {
     uint64_t shadow_seed;       /* seed of the random shadow residual                                 */
     void* X_device;             /* optional: receives the FP64 solution [nnzb][2][64][64] (device)   */
     void* stream;               /* hipStream_t or NULL                                                */
+    double inner_tolerance;     /* fp_bits 32: residual reduction asked of one FP32 solve (0: 1e-5)   */
+    int32_t echo;               /* >0: one line per inner solve on stderr                             */
+    int32_t reserved;
 } gfa_solve_opts;
 
 typedef struct {

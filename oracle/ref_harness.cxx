@@ -76,6 +76,7 @@ void* ref_plan_create(uint32_t const ng[3], int8_t const bc[3], double const hg[
     uint32_t ngc[3] = {ng[0], ng[1], ng[2]};
     auto const stat = green_function::construct_Green_function(rp->p, ngc, bcc, hgc, rp->xyzZinso, echo, 1);
     if (stat) { delete rp; return nullptr; }
+    rp->p.echo = 0;  // the plan's own log level defaults to 9 (include/action_plan.hxx:249)
     // replace the row-indexed potential by a cell-indexed one (read by the kernels via veff_index)
     size_t const ncell = size_t(rp->nb[0])*rp->nb[1]*rp->nb[2];
     free_memory(rp->p.Veff[0]);

@@ -1,0 +1,96 @@
+"""The C-ABI library: loads without a GPU, exports every symbol include/gfa.h declares, validates its
+arguments and reports errors through status codes (never exits).  Host-side only: no device work."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from angstromcube_amd import gfa as G
+from angstromcube_amd import synthetic as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "gfa.h")).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(gfa_\w+)\(", text, flags=re.M)))
+
+
+def test_header_declares_the_boundary():
+    syms = declared_symbols()
+    assert "gfa_action_multiply" in syms and "gfa_solve" in syms and "gfa_plan_create" in syms
+    assert set(syms) == set(G.C_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(G.LIB_PATH)
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+
+
+def test_plan_without_gpu_and_queries():
+    sysm = S.small_periodic()
+    p = G.GreenPlan(sysm, r_trunc=6.0)
+    assert p.nnzb == 512 and p.nCols == 1 and p.nRows == 512
+    assert tuple(p.sizes.bc) == (1, 1, 1)  # untruncated: stays periodic
+    p2 = G.GreenPlan(G_wrap := S.make_system(3, (16, 16, 16), ngauss=2, natoms=2, numax=1), r_trunc=4.0)
+    assert tuple(p2.sizes.bc) == (5, 5, 5)  # the sphere fits: wrapped
+    del G_wrap
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (dict(ng=(30, 32, 32)), "multiple of 4"),
+    (dict(r_trunc=-1.0), "non-negative"),
+    (dict(fd_range=(8, 4, 4), bc=(1, 1, 1)), "periodic"),
+    (dict(fd_range=(6, 4, 4)), "range must be 4 or 8"),
+    (dict(bc=(4, 0, 0)), "boundary condition"),
+])
+def test_plan_create_rejects_bad_input(bad, msg):
+    sysm = S.make_system(1, (4, 4, 4), ngauss=1, natoms=1, numax=1, bc=bad.get("bc", (0, 0, 0)))
+    d = G.PlanDesc()
+    ng = bad.get("ng", sysm.ng)
+    for k in range(3):
+        d.ng[k] = ng[k]
+        d.bc[k] = bad.get("bc", sysm.bc)[k]
+        d.hg[k] = sysm.h
+        d.fd_range[k] = bad.get("fd_range", (4, 4, 4))[k]
+        d.nsources[k] = 1
+    xyz = np.ascontiguousarray(sysm.xyzZinso)
+    d.xyzZinso = xyz.ctypes.data_as(C.POINTER(C.c_double))
+    d.natoms = 1
+    d.r_trunc = bad.get("r_trunc", 3.0)
+    d.r_proj = 6.0
+    h = C.c_void_p()
+    st = G.lib().gfa_plan_create(C.byref(d), C.byref(h))
+    assert st == 1 and not h.value
+    assert msg in G.lib().gfa_last_error().decode()
+
+
+def test_errors_are_status_codes():
+    L = G.lib()
+    assert L.gfa_plan_create(None, None) == 1
+    assert L.gfa_plan_destroy(None) == 0
+    assert L.gfa_set_energy(None, 0.0, 0.0, 1.0) == 1
+    sysm = S.make_system(1, (4, 4, 4), ngauss=1, natoms=1, numax=1, bc=(0, 0, 0))
+    p = G.GreenPlan(sysm, r_trunc=3.0)
+    assert L.gfa_set_energy(p.h, 0.0, 0.1, 0.0) == 1          # dVol must be positive
+    k = (C.c_double * 3)(0.7, 0, 0)
+    assert L.gfa_set_kpoint(p.h, k) == 1                      # |k| > 1/2
+    m = np.zeros((2, 3, 3))
+    assert L.gfa_set_atom_matrices(p.h, 0, m.ctypes.data_as(C.POINTER(C.c_double)), 3) == 1  # nc mismatch
+    assert L.gfa_set_atom_matrices(p.h, 5, m.ctypes.data_as(C.POINTER(C.c_double)), 4) == 1  # no such atom
+    o = G.SolveOpts()
+    o.fp_bits = 64
+    o.tolerance = 1e-9
+    assert L.gfa_solve(p.h, C.byref(o), None, None) == 3      # no energy set: state error
+    assert "energy" in L.gfa_last_error().decode()
+
+
+def test_export_rejects_small_buffers():
+    sysm = S.small_periodic()
+    p = G.GreenPlan(sysm, r_trunc=6.0)
+    buf = np.zeros(10, np.uint32)
+    st = G.lib().gfa_plan_export(p.h, G.EXPORT_ID["ROW_START"], buf.ctypes.data_as(C.c_void_p), buf.nbytes)
+    assert st == 1 and "too small" in G.lib().gfa_last_error().decode()
