@@ -84,7 +84,7 @@ __global__ __launch_bounds__(128) void k_sho_project(
     uint32_t const* __restrict__ bucket, uint32_t const* __restrict__ chunk_coef,
     uint32_t const* __restrict__ prj_blocks, uint32_t const* __restrict__ rowindx,
     float const* __restrict__ row_pos, double const* __restrict__ image_pos, double hx, double hy, double hz,
-    double rproj) {
+    double rproj, uint8_t const* __restrict__ col_active) {
     __shared__ real_t H[L + 1][3][4];
     __shared__ float xi2[3][4];
     __shared__ double apos[4];
@@ -92,6 +92,7 @@ __global__ __launch_bounds__(128) void k_sho_project(
     __shared__ double hgrid[4];
     uint32_t const ich = bucket[blockIdx.x];
     ProjChunk const ch = chunks[ich];
+    if (col_active && !col_active[ch.col]) return;  // a retired column group
     int const tid = threadIdx.x, j = tid & 63, p = tid >> 6;
     if (tid < 4) apos[tid] = image_pos[4 * size_t(ch.image) + tid];
     if (tid == 0) { hgrid[0] = hx; hgrid[1] = hy; hgrid[2] = hz; hgrid[3] = rproj; }
@@ -170,9 +171,11 @@ __global__ __launch_bounds__(64) void k_sho_atoms(
     uint32_t const* __restrict__ sum_index, uint32_t const* __restrict__ image_starts,
     int8_t const* __restrict__ atom_lmax, double const* __restrict__ atom_mat,
     uint64_t const* __restrict__ atom_mat_off, double const* __restrict__ image_phase, uint32_t nCols,
-    double* __restrict__ cscratch, uint32_t const* __restrict__ atom_starts) {
+    double* __restrict__ cscratch, uint32_t const* __restrict__ atom_starts, double const* __restrict__ E_col,
+    uint8_t const* __restrict__ col_active) {
     extern __shared__ double lds[];
     uint32_t const a = blockIdx.x, c = blockIdx.y;
+    if (col_active && !col_active[c]) return;  // a retired column group
     int const j = threadIdx.x;
     int const L = atom_lmax[a];
     int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
@@ -196,12 +199,15 @@ __global__ __launch_bounds__(64) void k_sho_atoms(
             cim[s * 64 + j] += ph_re * vi + ph_im * vr;
         }
     }
-    double const* M = atom_mat + atom_mat_off[a];  // [2][nc][nc]
-    // d = M c, complex double accumulation in ascending column order (include/green_dyadic.hxx:800-819)
+    double const* HS = atom_mat + atom_mat_off[a];  // [2][nc][nc]: h dVol, s dVol
+    double const Er = E_col[2 * c], Ei = E_col[2 * c + 1];
+    // d = M c with M = (h - E s) dVol of this column's energy, complex double accumulation in ascending
+    // column order (include/green_dyadic.hxx:800-819, src/green_function.cxx:97-98)
     for (int i = 0; i < nc; ++i) {
         double dr = 0, di = 0;
         for (int k = 0; k < nc; ++k) {
-            double const mr = M[i * nc + k], mi = M[nc * nc + i * nc + k];
+            double const hd = HS[i * nc + k], sd = HS[nc * nc + i * nc + k];
+            double const mr = hd - Er * sd, mi = -Ei * sd;
             double const cr = cre[k * 64 + j], ci = cim[k * 64 + j];
             dr += mr * cr - mi * ci;
             di += mr * ci + mi * cr;
@@ -241,9 +247,10 @@ struct LocalArgs {
     int8_t const* image_lmax;
     uint32_t const* image_starts;
     uint64_t nnzb;
+    uint64_t nitems;         // blocks to process (all, or the active ones) listed in order[]
     uint32_t nCols;
     uint32_t per_xcd;        // work-items per XCD in the XCD-contiguous mapping
-    real_t E_re, E_im;
+    double const* E_col;     // [nCols][2] energy of each column group
     real_t c8[3][5];         // 8th-order weights per direction
     real_t c16[3][9];        // 16th-order weights per direction
     int nh[3];               // neighbour blocks per side used in direction d (1 or 2)
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
     if (a.per_xcd) lin = uint64_t(blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
     uint64_t const item = lin / NSLICE;
     int const slice = int(lin % NSLICE);
-    if (item >= a.nnzb) return;  // uniform over the work-group
+    if (item >= a.nitems) return;  // uniform over the work-group
     uint64_t const blk = a.order[item];
     int const tid = threadIdx.x;
 
@@ -351,11 +358,13 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
         int32_t const iv = a.veff[blk];
         real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
         double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64;
+        uint32_t const colE = a.colindx[blk];
+        real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
 #pragma unroll
         for (int k = 0; k < 2 * 64 * JW / T; ++k) {
             int const e = k * T + tid, jl = e % JW, row = e / JW, p = row >> 6, i = row & 63;
-            real_t const Vim = a.E_im * real_t(1 - 2 * p);
-            ys[e] = (real_t(Vb[i]) * vm - a.E_re) * xs[e] + Vim * xs[((1 - p) * 64 + i) * JW + jl];
+            real_t const Vim = E_im * real_t(1 - 2 * p);
+            ys[e] = (real_t(Vb[i]) * vm - E_re) * xs[e] + Vim * xs[((1 - p) * 64 + i) * JW + jl];
         }
     }
 
@@ -483,13 +492,15 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
 // SHO add-back of atom images with numax 6 or 7, y += P d, for the (block, image) pairs listed in
 // pairs[]: one work-group of 256 threads = (column j, plane p, z plane zq) per pair.  Runs after k_local.
 template <typename real_t>
-__global__ __launch_bounds__(512) void k_sho_add_wide(LocalArgs<real_t> a, uint32_t const* __restrict__ pairs) {
+__global__ __launch_bounds__(512) void k_sho_add_wide(LocalArgs<real_t> a, uint32_t const* __restrict__ pairs,
+                                                      uint8_t const* __restrict__ col_active) {
     __shared__ real_t H[8][3][4];
     __shared__ float xi2[3][4];
     __shared__ double apos[4];
     __shared__ float cpos[4];
     __shared__ double hgrid[4];
     uint32_t const blk = pairs[2 * blockIdx.x], img = pairs[2 * blockIdx.x + 1];
+    if (col_active && !col_active[a.colindx[blk]]) return;
     int const tid = threadIdx.x, j = tid & 63, p = (tid >> 6) & 1, zq = tid >> 7;
     int const L = a.image_lmax[img];
     if (tid < 4) { apos[tid] = a.image_pos[4 * size_t(img) + tid]; hgrid[tid] = a.hgrid[tid]; }
@@ -689,7 +700,8 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
     }
 
     if ((s = dalloc(d, d.V, p.ncell() * 64, err))) return s;
-    if ((s = dalloc(d, d.atom_mat, p.atom_matrix.size(), err))) return s;
+    if ((s = dalloc(d, d.atom_hs, p.atom_hs.size(), err))) return s;
+    if ((s = dalloc(d, d.E_col, size_t(2) * p.nCols, err))) return s;
     if ((s = dalloc(d, d.image_phase, p.image_phase.size(), err))) return s;
     d.uploaded = true;
     return 0;
@@ -697,8 +709,9 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
 
 int device_upload_values(Plan const& p, DevicePlan& d, std::string& err) {
     HIP_TRY(hipMemcpy(d.V, p.V_cell.data(), p.V_cell.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (!p.atom_matrix.empty())
-        HIP_TRY(hipMemcpy(d.atom_mat, p.atom_matrix.data(), p.atom_matrix.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!p.atom_hs.empty())
+        HIP_TRY(hipMemcpy(d.atom_hs, p.atom_hs.data(), p.atom_hs.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.E_col, p.E_col.data(), p.E_col.size() * sizeof(std::complex<double>), hipMemcpyHostToDevice));
     if (!p.image_phase.empty())
         HIP_TRY(hipMemcpy(d.image_phase, p.image_phase.data(), p.image_phase.size() * sizeof(double), hipMemcpyHostToDevice));
     return 0;
@@ -724,7 +737,7 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
             uint32_t const* bucket = d.bucket_chunks + d.bucket_start[L];
 #define LAUNCH_PRJ(LL) hipLaunchKernelGGL((k_sho_project<real_t, LL>), dim3(n), dim3(128), 0, stream, ws.partial, x, \
                 d.chunks, bucket, d.chunk_coef, d.prj_blocks, d.rowindx, d.row_pos, d.image_pos, \
-                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3])
+                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], d.col_active)
             switch (L) {
                 case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
                 case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;
@@ -740,8 +753,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
         size_t const lds = size_t(2) * std::min<uint32_t>(d.max_atom_nc, 64) * 64 * sizeof(double);
         hipLaunchKernelGGL((k_sho_atoms<real_t>), dim3(d.nAtoms, d.nCols), dim3(64), lds, stream, ws.dcoef,
                            ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.image_starts,
-                           d.atom_lmax, d.atom_mat, d.atom_mat_off, d.image_phase, d.nCols, d.cscratch,
-                           d.atom_starts);
+                           d.atom_lmax, d.atom_hs, d.atom_mat_off, d.image_phase, d.nCols, d.cscratch,
+                           d.atom_starts, d.E_col, d.col_active);
         HIP_TRY(hipGetLastError());
     }
     LocalArgs<real_t> a;
@@ -754,8 +767,9 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
         HIP_TRY(hipMemsetAsync(ws.zero, 0, NBLK * sizeof(real_t), stream));
     }
     a.zero = ws.zero;
-    a.order = d.order;
-    a.E_re = real_t(s.E_re); a.E_im = real_t(s.E_im);
+    a.order = d.active_order ? d.active_order : d.order;
+    a.nitems = d.active_order ? d.n_active : d.nnzb;
+    a.E_col = d.E_col;
     static int const c8i[5] = {-14350, 8064, -1008, 128, -9};
     static int const c16i[9] = {-924708642, 538137600, -94174080, 22830080, -5350800, 1053696, -156800, 15360, -735};
     for (int dd = 0; dd < 3; ++dd) {
@@ -769,7 +783,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     // work-groups are dealt round-robin to the 8 XCDs: give each XCD a contiguous range of
     // (block, column half) items so that neighbour blocks meet in the same L2
     constexpr int JW = LOCAL_JW;
-    uint64_t const items = (64 / JW) * d.nnzb;
+    if (a.nitems == 0) return 0;
+    uint64_t const items = (64 / JW) * a.nitems;
     uint64_t const per = (items + 7) / 8;
     a.per_xcd = uint32_t(per);
     uint64_t const grid = per * 8;
@@ -784,7 +799,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     }
     HIP_TRY(hipGetLastError());
     if (d.n_wide_pairs > 0) {
-        hipLaunchKernelGGL((k_sho_add_wide<real_t>), dim3(d.n_wide_pairs), dim3(512), 0, stream, a, d.wide_pairs);
+        hipLaunchKernelGGL((k_sho_add_wide<real_t>), dim3(d.n_wide_pairs), dim3(512), 0, stream, a, d.wide_pairs,
+                           d.col_active);
         HIP_TRY(hipGetLastError());
     }
     return 0;

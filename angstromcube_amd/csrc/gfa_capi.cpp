@@ -102,6 +102,10 @@ int gfa_plan_create(const gfa_plan_desc* desc, gfa_plan** plan) {
             if (!desc->sources) return fail(GFA_ERR_ARGUMENT, "null explicit source list");
             in.sources.assign(desc->sources, desc->sources + 3 * size_t(desc->nsources_explicit));
         }
+        if (desc->source_radii) {
+            if (desc->nsources_explicit <= 0) return fail(GFA_ERR_ARGUMENT, "per-source radii need an explicit source list");
+            in.source_radii.assign(desc->source_radii, desc->source_radii + size_t(desc->nsources_explicit));
+        }
         auto* p = new gfa_plan();
         p->echo = desc->echo;
         std::string err;
@@ -232,6 +236,20 @@ int gfa_set_energy(gfa_plan* plan, double E_re, double E_im, double dVol) {
         if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
         if (!(dVol > 0)) return fail(GFA_ERR_ARGUMENT, "dVol must be positive");
         plan->host.set_energy(std::complex<double>(E_re, E_im), dVol);
+        plan->energy_set = true;
+        plan->values_dirty = true;
+        return GFA_OK;
+    });
+}
+
+int gfa_set_energies(gfa_plan* plan, const double* E, double dVol) {
+    return guarded([&]() -> int {
+        if (!plan || !E) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (!(dVol > 0)) return fail(GFA_ERR_ARGUMENT, "dVol must be positive");
+        std::vector<std::complex<double>> Es(plan->host.nCols);
+        for (uint32_t c = 0; c < plan->host.nCols; ++c) Es[c] = std::complex<double>(E[2 * c], E[2 * c + 1]);
+        std::string err;
+        if (plan->host.set_energies(Es.data(), dVol, err)) return fail(GFA_ERR_ARGUMENT, err);
         plan->energy_set = true;
         plan->values_dirty = true;
         return GFA_OK;

@@ -69,7 +69,12 @@ struct DevicePlan {
 
     // per-system / per-(E,k) data
     double* V = nullptr;               // [ncell][64]
-    double* atom_mat = nullptr;        // concatenated [2][nc][nc]
+    double* atom_hs = nullptr;         // concatenated [2][nc][nc]: h dVol, s dVol
+    double* E_col = nullptr;           // [nCols][2] energy per column group
+    // retirement: when set, the action processes only the listed blocks and skips inactive columns
+    uint32_t* active_order = nullptr;  // [n_active] blocks of active column groups, in processing order
+    uint64_t n_active = 0;
+    uint8_t* col_active = nullptr;     // [nCols] 1 for an active column group
     uint64_t* atom_mat_off = nullptr;  // [nAtoms+1]
     double* image_phase = nullptr;     // [nai][2]
 

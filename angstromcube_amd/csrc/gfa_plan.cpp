@@ -79,6 +79,8 @@ int Plan::build(PlanInput const& in, std::string& err) {
         hg[d] = in.hg[d];
     }
     if (!(in.r_trunc >= 0)) { err = "the truncation radius must be non-negative"; return 1; }
+    for (double r : in.source_radii)
+        if (!(r >= 0)) { err = "the truncation radius must be non-negative"; return 1; }
     if (in.xyzZinso.size() % 8) { err = "xyzZinso must hold 8 numbers per atom"; return 1; }
     xyzZinso = in.xyzZinso;
     size_t const natoms = xyzZinso.size() / 8;
@@ -88,6 +90,7 @@ int Plan::build(PlanInput const& in, std::string& err) {
         if (!(xyzZinso[ia * 8 + 6] > 0)) { err = "SHO sigma must be positive"; return 1; }
     }
     r_trunc = in.r_trunc;
+    for (double r : in.source_radii) r_trunc = std::max(r_trunc, r);  // the largest sphere decides wrapping
     r_proj = in.r_proj;
 
     std::vector<std::vector<uint8_t>> sparsity;
@@ -108,6 +111,8 @@ int Plan::build(PlanInput const& in, std::string& err) {
         atom_mat_off[a + 1] = atom_mat_off[a] + 2ull * nc * nc;
     }
     atom_matrix.assign(atom_mat_off[nAtoms], 0.0);
+    atom_hs.assign(atom_mat_off[nAtoms], 0.0);
+    E_col.assign(nCols, std::complex<double>(0, 0));
     return 0;
 }
 
@@ -190,15 +195,20 @@ int Plan::build_bsr(PlanInput const& in, std::string& err,
     uint64_t const nbox3 = uint64_t(num_target[X]) * num_target[Y] * num_target[Z];
     if (nbox3 == 0 || nbox3 > (1ull << 31)) { err = "target box empty or too large"; return 1; }
 
-    double const r2trunc = sq(rtrunc), r2trunc_plus = sq(rtrunc_plus);
     double const r2block_circum = sq(r_block_circ * 3);
+    if (!in.source_radii.empty() && in.source_radii.size() != nCols) { err = "one radius per source block needed"; return 1; }
+    source_radius.assign(nCols, rtrunc);
+    for (size_t c = 0; c < in.source_radii.size(); ++c) source_radius[c] = std::max(0., in.source_radii[c]);
     double const* h = htrunc;
 
     sparsity.assign(nCols, std::vector<uint8_t>(nbox3, 0));
     std::vector<std::vector<uint16_t>> cols_of_target(nbox3);
-    std::vector<int32_t> diag_tag(nbox3, -1);
+    std::vector<uint64_t> diag_i3(nCols, ~uint64_t(0));  // target box index of each column's own block
     for (uint32_t c = 0; c < nCols; ++c) {
         int32_t const* s = &source_coords[3 * c];
+        // this source's sphere (the reference has one radius for all; src/green_function.cxx:580-581, 540)
+        double const rc = source_radius[c];
+        double const r2trunc = sq(rc), r2trunc_plus = sq(rc + 2 * r_block_circ);
         int32_t bfirst[3], blast[3];
         for (int d = 0; d < 3; ++d) {
             if (BC_ISOLATED == bc[d] || BC_PERIODIC == bc[d]) {
@@ -235,7 +245,7 @@ int Plan::build_bsr(PlanInput const& in, std::string& err,
                 if (sparsity[c][i3]) { err = "a target block was hit twice"; return 1; }
                 sparsity[c][i3] = 1;
                 cols_of_target[i3].push_back(uint16_t(c));
-                if (0 == bx && 0 == by && 0 == bz) { diag_tag[i3] = int32_t(c); diag_hit = true; }
+                if (0 == bx && 0 == by && 0 == bz) { diag_i3[c] = i3; diag_hit = true; }
             }
         }
         if (!diag_hit) { err = "a source block is not inside its own truncation sphere"; return 1; }
@@ -285,7 +295,7 @@ int Plan::build_bsr(PlanInput const& in, std::string& err,
             colindx[inz] = cl[k];
             rowindx[inz] = iRow;
             veff_index[inz] = vidx;
-            if (diag_tag[i3] == int32_t(cl[k])) subset[cl[k]] = inz;
+            if (diag_i3[cl[k]] == i3) subset[cl[k]] = inz;  // columns may share a source block
         }
         ++iRow;
     }
@@ -566,7 +576,17 @@ int Plan::set_atom_matrices(int ia, double const* hs, int nc, std::string& err) 
 void Plan::set_energy(std::complex<double> E_, double dVol_) {
     E = E_;
     dVol = dVol_;
+    E_col.assign(nCols, E_);
     update_atom_matrices();
+}
+
+int Plan::set_energies(std::complex<double> const* Es, double dVol_, std::string& err) {
+    if (!Es) { err = "null energies"; return 1; }
+    dVol = dVol_;
+    E_col.assign(Es, Es + nCols);
+    E = E_col[0];
+    update_atom_matrices();
+    return 0;
 }
 
 // (h - E s) dVol per contributing atom (src/green_function.cxx:64-109)
@@ -575,12 +595,19 @@ void Plan::update_atom_matrices() {
         int const nc = nsho(atom_lmax[a]);
         double* M = &atom_matrix[atom_mat_off[a]];
         auto const& hs = hs_scaled[atom_orig[a]];
-        if (hs.empty()) { std::fill(M, M + 2 * nc * nc, 0.0); continue; }
+        if (hs.empty()) {
+            std::fill(M, M + 2 * nc * nc, 0.0);
+            std::fill(atom_hs.begin() + atom_mat_off[a], atom_hs.begin() + atom_mat_off[a + 1], 0.0);
+            continue;
+        }
         double const* hm = hs.data();
         double const* sm = hs.data() + nc * nc;
+        double* HS = &atom_hs[atom_mat_off[a]];
         for (int i = 0; i < nc * nc; ++i) {
             M[i] = (scale_H * hm[i] - E.real() * sm[i]) * dVol;
             M[i + nc * nc] = (-E.imag() * sm[i]) * dVol;
+            HS[i] = scale_H * hm[i] * dVol;        // the device forms (h - E_c s) dVol per column c
+            HS[i + nc * nc] = sm[i] * dVol;
         }
     }
 }

@@ -48,6 +48,7 @@ struct PlanInput {
     int fd_range[3] = {0, 0, 0};      // 0: default (8, reduced to 4 in periodic directions)
     int nsources[3] = {1, 1, 1};      // box of source blocks at the cell centre, <0: all
     std::vector<int32_t> sources;     // optional explicit source block coordinates [n][3]
+    std::vector<double> source_radii; // optional truncation radius per source block (batched instances)
     double scale_T = 1.0;             // hamiltonian.scale.kinetic
 };
 
@@ -67,6 +68,7 @@ public:
     uint32_t nRows = 0, nCols = 0;
     uint64_t nnzb = 0;
     std::vector<int32_t> source_coords;     // [nCols][3] global block coordinates
+    std::vector<double> source_radius;      // [nCols] truncation radius of each column's sphere
     std::vector<int64_t> global_source_ids; // [nCols] bit-interleaved ids (include/global_coordinates.hxx:20-48)
     std::vector<uint32_t> row_start;        // [nRows+1]
     std::vector<uint16_t> colindx;          // [nnzb]
@@ -98,8 +100,10 @@ public:
     std::vector<double> xyzZinso;
     std::vector<double> V_cell;                  // [ncell][64] cell-indexed local potential
     std::vector<std::vector<double>> hs_scaled;  // per original atom: [2][nc][nc] rescaled h, s
-    std::complex<double> E{0, 0};
+    std::complex<double> E{0, 0};                 // energy of column 0 (the only one unless batched)
+    std::vector<std::complex<double>> E_col;      // [nCols] energy of each column group
     double dVol = 0, scale_H = 1;
+    std::vector<double> atom_hs;                  // per contributing atom [2][nc][nc]: h dVol, s dVol (at atom_mat_off)
     std::vector<double> atom_matrix;             // per contributing atom, [2][nc][nc] at atom_mat_off
     std::vector<uint64_t> atom_mat_off;          // [nAtoms+1]
     double phase[3][2][2] = {{{1, 0}, {1, 0}}, {{1, 0}, {1, 0}}, {{1, 0}, {1, 0}}};
@@ -113,6 +117,7 @@ public:
     int set_potential(double const* V_grid, std::string& err);              // V[ng_z][ng_y][ng_x]
     int set_atom_matrices(int ia, double const* hs, int nc, std::string& err); // raw L2-normalised [2][nc][nc]
     void set_energy(std::complex<double> E, double dVol);
+    int set_energies(std::complex<double> const* E, double dVol, std::string& err);  // one per column
     int set_kpoint(double const k[3], std::string& err);
     void update_atom_matrices();
 

@@ -24,7 +24,7 @@ EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS",
 EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
-             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply",
              "gfa_solve", "gfa_last_error"]
 
 
@@ -37,7 +37,7 @@ class PlanDesc(C.Structure):
                 ("xyzZinso", C.POINTER(C.c_double)), ("natoms", C.c_int32), ("echo", C.c_int32),
                 ("r_trunc", C.c_double), ("r_proj", C.c_double), ("fd_range", C.c_int32 * 3),
                 ("nsources", C.c_int32 * 3), ("sources", C.POINTER(C.c_int32)), ("nsources_explicit", C.c_int32),
-                ("reserved1", C.c_int32), ("scale_T", C.c_double)]
+                ("reserved1", C.c_int32), ("scale_T", C.c_double), ("source_radii", C.POINTER(C.c_double))]
 
 
 class Sizes(C.Structure):
@@ -112,7 +112,7 @@ class GreenPlan:
     """
 
     def __init__(self, system, r_trunc, fd_range=(4, 4, 4), nsources=(1, 1, 1), sources=None, r_proj=6.0,
-                 echo=0):
+                 echo=0, source_radii=None):
         L = lib()
         d = PlanDesc()
         for k in range(3):
@@ -132,6 +132,11 @@ class GreenPlan:
             self._src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32).reshape(-1, 3))
             d.sources = self._src.ctypes.data_as(C.POINTER(C.c_int32))
             d.nsources_explicit = self._src.shape[0]
+        if source_radii is not None:
+            self._radii = np.ascontiguousarray(source_radii, dtype=np.float64)
+            if sources is None or self._radii.shape != (self._src.shape[0],):
+                raise GfaError("one truncation radius per explicit source block is needed")
+            d.source_radii = _dptr(self._radii)
         h = C.c_void_p()
         _check(L.gfa_plan_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -207,13 +212,25 @@ class GreenPlan:
         dVol = self.system.dvol if dVol is None else dVol
         _check(lib().gfa_set_energy(self.h, float(E.real), float(E.imag), float(dVol)))
 
+    def update_energies(self, E, dVol=None):
+        """One energy per column group (batched contour instances)."""
+        dVol = self.system.dvol if dVol is None else dVol
+        e = np.zeros((self.nCols, 2))
+        E = np.broadcast_to(np.asarray(E, dtype=complex), (self.nCols,))
+        e[:, 0], e[:, 1] = E.real, E.imag
+        _check(lib().gfa_set_energies(self.h, _dptr(e), float(dVol)))
+
     def update_phases(self, k):
         kk = (C.c_double * 3)(*k)
         _check(lib().gfa_set_kpoint(self.h, kk))
 
     def set_inputs(self, E, k=(0.0, 0.0, 0.0)):
+        """E: one energy, or one per column group."""
         self.update_potential()
-        self.update_energy_parameter(E)
+        if np.ndim(E) == 0:
+            self.update_energy_parameter(E)
+        else:
+            self.update_energies(E)
         self.update_phases(k)
 
     # ---- device work ----

@@ -46,6 +46,9 @@ typedef struct {
     int32_t nsources_explicit;
     int32_t reserved1;
     double scale_T;             /* hamiltonian.scale.kinetic (1)                                   */
+    const double* source_radii; /* optional [nsources_explicit] truncation radius of each source block
+                                 * (batched (E, radius) instances); NULL: r_trunc for all. Sources may
+                                 * repeat, each repetition is a column group of its own.            */
 } gfa_plan_desc;
 
 typedef struct {
@@ -120,6 +123,9 @@ int gfa_plan_export(const gfa_plan* plan, int what, void* dst, uint64_t dst_byte
 int gfa_set_potential(gfa_plan* plan, const double* V);
 int gfa_set_atom_matrices(gfa_plan* plan, int32_t ia, const double* hs, int32_t nc);
 int gfa_set_energy(gfa_plan* plan, double E_re, double E_im, double dVol);
+/* one energy per column group, E[nCols][2] (re, im): the batched form of update_energy_parameter for
+ * instances of an energy contour (src/energy_contour.cxx:364-380 loops over the points one at a time) */
+int gfa_set_energies(gfa_plan* plan, const double* E, double dVol);
 int gfa_set_kpoint(gfa_plan* plan, const double k[3]);
 
 /* ---- the action y = (H - E S) x on device pointers; replaces action_t::multiply

@@ -1,0 +1,124 @@
+"""Batched instances: one plan holding several (source block, truncation radius, energy) instances as column
+groups.  Each column group must behave exactly like a plan built for that instance alone: same target blocks,
+same finite-difference and projector structure (checked through the CPU oracle's action), and on the GPU the
+batched action with one energy per column group must equal the oracle of every instance."""
+import numpy as np
+import pytest
+
+from angstromcube_amd import gfa as G
+from angstromcube_amd import synthetic as S
+from oracle import oracle as O
+
+SOURCES = [(8, 8, 8), (8, 8, 8), (7, 8, 9)]     # a repeated source block: two instances of one column
+RADII = [3.0, 5.0, 4.0]
+ENERGIES = [-0.5 + 0.05j, -0.2 + 0.3j, -0.8 + 0.01j]
+K = (0.0, 0.0, 0.0)
+
+
+def system():
+    return S.make_system(0xBA7C4, (16, 16, 16), ngauss=4, natoms=6, numax=3)
+
+
+def batched(sysm):
+    return G.GreenPlan(sysm, r_trunc=max(RADII), sources=SOURCES, source_radii=RADII)
+
+
+def single(sysm, c):
+    return G.GreenPlan(sysm, r_trunc=RADII[c], sources=[SOURCES[c]])
+
+
+def block_keys(plan, col):
+    """(target block coordinates) -> block index for the blocks of column group col."""
+    rows, cols, pos = plan.export("ROWINDX"), plan.export("COLINDX"), plan.export("ROW_POS")
+    sel = np.nonzero(cols == col)[0]
+    return {tuple(pos[rows[b], :3].astype(int)): int(b) for b in sel}
+
+
+def x_of(plan, seed=0xBA7C):
+    return S.hash_uniform(seed, plan.nnzb * 2 * 64 * 64).reshape(plan.nnzb, 2, 64, 64)
+
+
+def test_batched_plan_columns_equal_single_plans():
+    sysm = system()
+    pb = batched(sysm)
+    assert pb.nCols == 3
+    assert tuple(pb.sizes.bc) == (5, 5, 5)
+    sub = pb.export("SUBSET")
+    assert len(set(sub.tolist())) == 3          # a repeated source still gets its own diagonal block
+    cols, rows, pos = pb.export("COLINDX"), pb.export("ROWINDX"), pb.export("ROW_POS")
+    for c in range(3):
+        ps = single(sysm, c)
+        kb, ks = block_keys(pb, c), block_keys(ps, 0)
+        assert set(kb) == set(ks), c
+        assert cols[sub[c]] == c
+        own = tuple(int(v) for v in np.array(SOURCES[c]) - 8)
+        assert tuple(pos[rows[sub[c]], :3].astype(int)) == own
+
+
+def test_batched_radius_must_match_source_count():
+    sysm = system()
+    with pytest.raises(G.GfaError, match="radius"):
+        G.GreenPlan(sysm, r_trunc=5.0, sources=SOURCES, source_radii=RADII[:2])
+
+
+def test_batched_oracle_columns_equal_single_instances():
+    """Column group c of the batched plan applies the same operator as instance c's own plan."""
+    sysm = system()
+    pb = batched(sysm)
+    xb = x_of(pb)
+    for c in range(3):
+        yb = O.Oracle(pb, ENERGIES[c], K).action(xb)
+        ps = single(sysm, c)
+        kb, ks = block_keys(pb, c), block_keys(ps, 0)
+        xs = np.zeros((ps.nnzb, 2, 64, 64))
+        for key, b in ks.items():
+            xs[b] = xb[kb[key]]
+        ys = O.Oracle(ps, ENERGIES[c], K).action(xs)
+        err = max(np.abs(yb[kb[key]] - ys[b]).max() for key, b in ks.items())
+        assert err <= 1e-13 * np.abs(ys).max(), (c, err)
+
+
+def oracle_batched(plan, x):
+    """Per-column-energy action from single-energy oracle runs: columns never couple."""
+    cols = plan.export("COLINDX")
+    y = np.zeros_like(x)
+    for c, E in enumerate(ENERGIES):
+        yc = O.Oracle(plan, E, K).action(x)
+        y[cols == c] = yc[cols == c]
+    return y
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fp_bits,rtol", [(64, 1e-12), (32, 2e-5)])
+def test_hip_batched_action_matches_oracle(device, fp_bits, rtol):
+    import torch
+    sysm = system()
+    pb = batched(sysm)
+    pb.set_inputs(np.array(ENERGIES), K)
+    x = x_of(pb)
+    dt = torch.float64 if fp_bits == 64 else torch.float32
+    xd = torch.from_numpy(x).to(device=device, dtype=dt)
+    yd = torch.empty_like(xd)
+    print(f"batched action fp{fp_bits}: nnzb {pb.nnzb}", flush=True)
+    pb.multiply(yd, xd, fp_bits=fp_bits)
+    torch.cuda.synchronize()
+    ref = oracle_batched(pb, xd.double().cpu().numpy())
+    got = yd.double().cpu().numpy()
+    assert np.abs(got - ref).max() <= rtol * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_hip_batched_solve_matches_single_solves(device):
+    """The batched FP64 solve gives every instance the density of its own solve, to the solve tolerance."""
+    sysm = system()
+    pb = batched(sysm)
+    pb.set_inputs(np.array(ENERGIES), K)
+    print(f"batched solve: nnzb {pb.nnzb}", flush=True)
+    rho_b, st = pb.solve(fp_bits=64, tolerance=1e-9, max_iterations=3000)
+    assert st["converged_columns"] == st["total_columns"] == 192
+    for c in range(3):
+        ps = single(sysm, c)
+        ps.set_inputs(ENERGIES[c], K)
+        rho_s, st_s = ps.solve(fp_bits=64, tolerance=1e-9, max_iterations=3000)
+        assert st_s["converged_columns"] == 64
+        assert np.abs(rho_b[c] - rho_s[0]).max() <= 1e-6 * np.abs(rho_s[0]).max(), c
