@@ -51,7 +51,7 @@ int ensure_device(gfa_plan* p) {
         if (int s = gfa::device_upload_plan(p->host, p->dev, err)) return fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
     }
     if (p->values_dirty) {
-        if (int s = gfa::device_upload_values(p->host, p->dev, err)) return fail(GFA_ERR_DEVICE, err);
+        if (gfa::device_upload_values(p->host, p->dev, err)) return fail(GFA_ERR_DEVICE, err);
         p->values_dirty = false;
     }
     return GFA_OK;

@@ -42,6 +42,7 @@ struct DevicePlan {
     int32_t* veff = nullptr;           // [nnzb]
     int32_t* nbr = nullptr;            // [3][2][nh_max][nnzb] neighbour codes
     uint32_t* order = nullptr;         // [nnzb] processing order of k_local
+    std::vector<uint32_t> h_order;     // host copy of order
     float* row_pos = nullptr;          // [nRows][4]
     uint32_t* add_start = nullptr;     // [nnzb+1]
     uint32_t* add_index = nullptr;

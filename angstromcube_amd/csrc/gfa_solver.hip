@@ -6,11 +6,21 @@
 // library (call site include/green_action.hxx:216, tfqmrgpu::solve(action, buffer, 1e-9, maxiter, 0, true)),
 // which likewise keeps per-column scalars and FP64 inner products; its source is not available, so the
 // iterate sequence is this file's own: a seeded random shadow residual, per-column convergence on the
-// TFQMR residual bound tau*sqrt(m+1) <= tol*||b||, frozen columns once converged or broken down.
+// TFQMR residual bound tau*sqrt(m+2) <= tol*||b||, frozen columns once converged or broken down.
 //
-// All scalars live on the device; one half-step is a fixed sequence of launches with no host
-// synchronisation, and the host polls the column status every few half-steps.  Inner products are
-// reduced per block, then per column over fixed segments of blocks, so results are deterministic.
+// Layout of the work: one full TFQMR step (an even and an odd half-step, two actions) is
+//     alpha | K_even | theta_e | A un | K_odd | theta_o | K_dx2 | A u | K_vnext
+// where the K_* kernels stream the vectors of one segment (<= SEG blocks of one column group) per
+// work-group and leave their inner-product partials per segment; the scalar kernels fold the segments of
+// a column in a fixed order, so every reduction is deterministic.  The two x updates of a step are
+// deferred into one pass (K_dx2), the shadow product of v is fused into its update (K_vnext).
+// The solution x is always accumulated in FP64, also when the vectors are FP32: the FP32 rounding of x
+// would otherwise limit the true residual to about eps32 * ||A|| * ||x||.
+//
+// Retirement: a column group (one source block, 64 columns) whose columns have all converged leaves the
+// active set; the vector kernels, reductions and the action then skip its blocks.  The iteration sequence
+// of the active set is captured once as a HIP graph and replayed; the half-step counter lives on the
+// device so that replays need no new arguments.  The host polls the column status every few half-steps.
 #include "gfa_solver.hpp"
 
 #include <algorithm>
@@ -34,321 +44,425 @@ namespace {
     } while (0)
 
 constexpr int NBLK = 2 * 64 * 64;
-constexpr int SEG = 32;  // blocks per reduction segment
+constexpr int SEG = 16;  // blocks per segment (work-group of the vector kernels)
 
 // per-column scalar fields, stored field-major: S[field * ncol + q]
-enum Field { RHO_RE, RHO_IM, ALPHA_RE, ALPHA_IM, THETA, TAU, ETA_RE, ETA_IM, CD_RE, CD_IM,
-             BETA_RE, BETA_IM, BNORM, STATUS, TOL, NFIELDS };
+enum Field { RHO_RE, RHO_IM, ALPHA_RE, ALPHA_IM, THETA, TAU, ETAE_RE, ETAE_IM, ETAO_RE, ETAO_IM,
+             CDE_RE, CDE_IM, CDO_RE, CDO_IM, BETA_RE, BETA_IM, BNORM, STATUS, TOL, MSTEP, NFIELDS };
 enum Status { ACTIVE = 0, CONVERGED = 1, BREAKDOWN = 2 };
 
-struct SolverWs {
-    bool ready = false;
-    uint64_t nnzb = 0;
-    uint32_t ncol = 0;             // 64 * nCols
-    uint32_t nseg = 0;
-    // structure
-    uint32_t* col_blocks = nullptr;  // blocks grouped by column
-    uint32_t* seg = nullptr;         // [nseg][3]: column, begin, end into col_blocks
-    uint32_t* col_seg = nullptr;     // [nCols+1] segment range per column
-    uint8_t* diag = nullptr;         // [nnzb] 1 if the block is the diagonal block of its column
-    // reductions
-    double* part = nullptr;          // [nnzb][3][64]
-    double* segpart = nullptr;       // [nseg][3][64]
-    double* red = nullptr;           // [3][ncol]
-    double* S = nullptr;             // [NFIELDS][ncol]
-    double* S_host = nullptr;        // pinned copy of the status field
-    double* res = nullptr;           // [ncol] true residual norms
-    // vectors, per precision
-    void* v64[12] = {nullptr};   // 0 x, 1-7 TFQMR work vectors, 8 x64, 9 y64, 10 rhs64
-    void* v32[12] = {nullptr};   // 0 x, 1-7 TFQMR work vectors, 8 rhs32
-    double* tol_abs = nullptr;   // [ncol] absolute per-column tolerances of refinement solves
-    std::vector<void*> allocations;
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+template <typename T> struct Pk;
+template <> struct Pk<float> { static constexpr int V = 4; typedef f4 t; };
+template <> struct Pk<double> { static constexpr int V = 2; typedef d2 t; };
+
+// thread mapping of the vector kernels: 256 threads, each V consecutive columns j0..j0+V-1 of rows
+// i = pass*RPP + r, for both planes
+template <typename T> struct Map {
+    static constexpr int V = Pk<T>::V;
+    static constexpr int JG = 64 / V;      // column groups
+    static constexpr int RPP = 256 / JG;   // rows per pass
+    static constexpr int NPASS = 64 / RPP;
 };
 
-template <typename real_t> struct Cplx { real_t re, im; };
-
-// ------------------------------------------------------------------------------------------------
-// reductions
-// ------------------------------------------------------------------------------------------------
-// combine the 4 partial sums of a column j held by threads j, j+64, j+128, j+192 and store them
-template <int NC>
-__device__ __forceinline__ void store_partials(double (&acc)[NC], double* __restrict__ part, uint64_t blk) {
-    __shared__ double sh[NC][256];
-    int const tid = threadIdx.x;
+template <typename T>
+__device__ __forceinline__ void ld(T (&a)[Pk<T>::V], T const* p) {
+    typename Pk<T>::t const v = *reinterpret_cast<typename Pk<T>::t const*>(p);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sh[c][tid] = acc[c];
+    for (int t = 0; t < Pk<T>::V; ++t) a[t] = v[t];
+}
+template <typename T>
+__device__ __forceinline__ void st(T* p, T const (&a)[Pk<T>::V]) {
+    typename Pk<T>::t v;
+#pragma unroll
+    for (int t = 0; t < Pk<T>::V; ++t) v[t] = a[t];
+    *reinterpret_cast<typename Pk<T>::t*>(p) = v;
+}
+// V doubles (V = 2 or 4) as 16-byte pieces
+template <int V>
+__device__ __forceinline__ void ldd(double (&a)[V], double const* p) {
+#pragma unroll
+    for (int h = 0; h < V / 2; ++h) {
+        d2 const v = *reinterpret_cast<d2 const*>(p + 2 * h);
+        a[2 * h] = v[0]; a[2 * h + 1] = v[1];
+    }
+}
+template <int V>
+__device__ __forceinline__ void std_(double* p, double const (&a)[V]) {
+#pragma unroll
+    for (int h = 0; h < V / 2; ++h) {
+        d2 v; v[0] = a[2 * h]; v[1] = a[2 * h + 1];
+        *reinterpret_cast<d2*>(p + 2 * h) = v;
+    }
+}
+
+struct SegArgs {
+    uint32_t const* act;         // [grid] active segments (global segment numbers)
+    uint32_t const* seg;         // [nseg][3]: column group, begin, end into col_blocks
+    uint32_t const* col_blocks;  // blocks grouped by column group
+    double* segpart;             // [nseg][3][64], indexed by global segment number
+};
+
+// fold the per-thread partials acc[c][t] (columns j0+t) of a work-group into segpart[s][c][j]
+template <typename T, int NC>
+__device__ __forceinline__ void store_partials(double (&acc)[NC][Pk<T>::V], double* __restrict__ segpart, uint32_t s) {
+    using M = Map<T>;
+    __shared__ double sh[NC][M::RPP][64];
+    int const tid = threadIdx.x, r = tid / M::JG, j0 = (tid % M::JG) * M::V;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int t = 0; t < M::V; ++t) sh[c][r][j0 + t] = acc[c][t];
     __syncthreads();
     if (tid < 64) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            double const s = (sh[c][tid] + sh[c][tid + 64]) + (sh[c][tid + 128] + sh[c][tid + 192]);
-            part[(blk * 3 + c) * 64 + tid] = s;
+            double sum = 0;
+            for (int k = 0; k < M::RPP; ++k) sum += sh[c][k][tid];
+            segpart[(uint64_t(s) * 3 + c) * 64 + tid] = sum;
         }
     }
 }
-
-__global__ __launch_bounds__(64) void k_seg_reduce(double* __restrict__ segpart, double const* __restrict__ part,
-                                                   uint32_t const* __restrict__ seg, uint32_t const* __restrict__ col_blocks,
-                                                   int nc) {
-    uint32_t const s = blockIdx.x;
-    uint32_t const b0 = seg[3 * s + 1], b1 = seg[3 * s + 2];
-    int const j = threadIdx.x;
-    for (int c = 0; c < nc; ++c) {
-        double acc = 0;
-        for (uint32_t b = b0; b < b1; ++b) acc += part[(uint64_t(col_blocks[b]) * 3 + c) * 64 + j];
-        segpart[(uint64_t(s) * 3 + c) * 64 + j] = acc;
-    }
-}
-
-__global__ __launch_bounds__(64) void k_col_reduce(double* __restrict__ red, double const* __restrict__ segpart,
-                                                   uint32_t const* __restrict__ col_seg, uint32_t ncol, int nc) {
-    uint32_t const col = blockIdx.x;
-    int const j = threadIdx.x;
-    for (int c = 0; c < nc; ++c) {
-        double acc = 0;
-        for (uint32_t s = col_seg[col]; s < col_seg[col + 1]; ++s) acc += segpart[(uint64_t(s) * 3 + c) * 64 + j];
-        red[uint64_t(c) * ncol + col * 64 + j] = acc;
-    }
-}
-
-// ------------------------------------------------------------------------------------------------
-// element-wise kernels: one work-group per block, 256 threads, each thread 16 complex elements
-// (i = 4k + tid/64, j = tid % 64) of one column q = 64*colindx[blk] + j
-// ------------------------------------------------------------------------------------------------
-#define ELEMENT_LOOP for (int k = 0; k < 16; ++k)
-#define ELEM_RE(ptr) ptr[blk * NBLK + (4 * k + (tid >> 6)) * 64 + j]
-#define ELEM_IM(ptr) ptr[blk * NBLK + 4096 + (4 * k + (tid >> 6)) * 64 + j]
 
 __device__ __forceinline__ uint64_t elem_index(uint64_t blk, int p, int i, int j) {
     return ((blk * 2 + p) * 64 + i) * 64 + j;
 }
 
-// x = 0, d = 0, w = u = rhs; partials |rhs|^2, dot(rs, rhs)
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_init(real_t* __restrict__ x, real_t* __restrict__ d, real_t* __restrict__ w,
-                                              real_t* __restrict__ u, real_t const* __restrict__ rhs,
-                                              uint8_t const* __restrict__ diag, double* __restrict__ part,
-                                              uint64_t seed) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    double acc[3] = {0, 0, 0};
-    bool const dg = diag[blk];
-    ELEMENT_LOOP {
-        int const i = 4 * k + (tid >> 6);
-        real_t br, bi;
-        if (rhs) { br = ELEM_RE(rhs); bi = ELEM_IM(rhs); }
-        else { br = (dg && i == j) ? real_t(1) : real_t(0); bi = 0; }
-        ELEM_RE(x) = 0; ELEM_IM(x) = 0;
-        ELEM_RE(d) = 0; ELEM_IM(d) = 0;
-        ELEM_RE(w) = br; ELEM_IM(w) = bi;
-        ELEM_RE(u) = br; ELEM_IM(u) = bi;
-        double const sr = shadow_value(seed, elem_index(blk, 0, i, j)), si = shadow_value(seed, elem_index(blk, 1, i, j));
-        acc[0] += double(br) * br + double(bi) * bi;
-        acc[1] += sr * br + si * bi;   // conj(rs) * b
-        acc[2] += sr * bi - si * br;
+template <typename T> struct Cx { T re, im; };
+
+template <typename T, int N>
+__device__ __forceinline__ void coefs(Cx<T> (&c)[N], double const* __restrict__ S, uint32_t ncol, uint32_t q0, int fre) {
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        c[t].re = T(S[uint64_t(fre) * ncol + q0 + t]);
+        c[t].im = T(S[uint64_t(fre + 1) * ncol + q0 + t]);
     }
-    store_partials<3>(acc, part, blk);
 }
 
-// partial dot(rs, v)
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_dot_shadow(real_t const* __restrict__ v, double* __restrict__ part, uint64_t seed) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    double acc[2] = {0, 0};
-    ELEMENT_LOOP {
-        int const i = 4 * k + (tid >> 6);
-        double const vr = ELEM_RE(v), vi = ELEM_IM(v);
-        double const sr = shadow_value(seed, elem_index(blk, 0, i, j)), si = shadow_value(seed, elem_index(blk, 1, i, j));
-        acc[0] += sr * vr + si * vi;
-        acc[1] += sr * vi - si * vr;
+#define SEG_PROLOGUE                                                                     \
+    using M = Map<T>;                                                                    \
+    constexpr int V = M::V;                                                              \
+    uint32_t const s = g.act[blockIdx.x];                                                \
+    uint32_t const col = g.seg[3 * s], b0 = g.seg[3 * s + 1], b1 = g.seg[3 * s + 2];     \
+    int const tid = threadIdx.x, r = tid / M::JG, j0 = (tid % M::JG) * V;                \
+    uint32_t const q0 = col * 64 + j0;                                                   \
+    (void)q0; (void)r;
+#define OFF(p, i) ((uint64_t(blk) * 2 + (p)) * 4096 + uint64_t(i) * 64 + j0)
+
+// x untouched; d = 0, w = u = b (unit columns on the diagonal blocks if rhs is null);
+// partials |b|^2 and (r0*, b)
+template <typename T>
+__global__ __launch_bounds__(256) void k_init(T* __restrict__ d, T* __restrict__ w, T* __restrict__ u,
+                                              T const* __restrict__ rhs, uint8_t const* __restrict__ diag,
+                                              SegArgs g, uint64_t seed) {
+    SEG_PROLOGUE
+    double acc[3][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+        bool const dg = diag[blk];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T br[V], bi[V], z[V];
+            if (rhs) { ld(br, rhs + OFF(0, i)); ld(bi, rhs + OFF(1, i)); }
+            else {
+#pragma unroll
+                for (int t = 0; t < V; ++t) { br[t] = (dg && i == j0 + t) ? T(1) : T(0); bi[t] = 0; }
+            }
+#pragma unroll
+            for (int t = 0; t < V; ++t) z[t] = 0;
+            st(d + OFF(0, i), z); st(d + OFF(1, i), z);
+            st(w + OFF(0, i), br); st(w + OFF(1, i), bi);
+            st(u + OFF(0, i), br); st(u + OFF(1, i), bi);
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                double const sr = shadow_value(seed, elem_index(blk, 0, i, j0 + t));
+                double const si = shadow_value(seed, elem_index(blk, 1, i, j0 + t));
+                double const xr = br[t], xi = bi[t];
+                acc[0][t] += xr * xr + xi * xi;
+                acc[1][t] += sr * xr + si * xi;   // conj(r0*) b
+                acc[2][t] += sr * xi - si * xr;
+            }
+        }
     }
-    store_partials<2>(acc, part, blk);
+    store_partials<T, 3>(acc, g.segpart, s);
 }
 
-template <typename real_t>
-__device__ __forceinline__ Cplx<real_t> coef(double const* __restrict__ S, uint32_t ncol, uint32_t q, int fre) {
-    return Cplx<real_t>{real_t(S[uint64_t(fre) * ncol + q]), real_t(S[uint64_t(fre + 1) * ncol + q])};
+// partial (r0*, v)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_shadow(T const* __restrict__ v, SegArgs g, uint64_t seed) {
+    SEG_PROLOGUE
+    double acc[2][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T vr[V], vi[V];
+            ld(vr, v + OFF(0, i)); ld(vi, v + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                double const sr = shadow_value(seed, elem_index(blk, 0, i, j0 + t));
+                double const si = shadow_value(seed, elem_index(blk, 1, i, j0 + t));
+                acc[0][t] += sr * vr[t] + si * vi[t];
+                acc[1][t] += sr * vi[t] - si * vr[t];
+            }
+        }
+    }
+    store_partials<T, 2>(acc, g.segpart, s);
 }
 
 // even half-step: un = u - alpha v; w -= alpha Au; partial |w|^2
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_even(real_t* __restrict__ un, real_t const* __restrict__ u,
-                                              real_t const* __restrict__ v, real_t* __restrict__ w,
-                                              real_t const* __restrict__ Au, uint16_t const* __restrict__ colindx,
-                                              double const* __restrict__ S, uint32_t ncol, double* __restrict__ part) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    uint32_t const q = uint32_t(colindx[blk]) * 64 + j;
-    auto const a = coef<real_t>(S, ncol, q, ALPHA_RE);
-    double acc[1] = {0};
-    ELEMENT_LOOP {
-        real_t const ur = ELEM_RE(u), ui = ELEM_IM(u), vr = ELEM_RE(v), vi = ELEM_IM(v);
-        ELEM_RE(un) = ur - (a.re * vr - a.im * vi);
-        ELEM_IM(un) = ui - (a.re * vi + a.im * vr);
-        real_t const yr = ELEM_RE(Au), yi = ELEM_IM(Au);
-        real_t const wr = ELEM_RE(w) - (a.re * yr - a.im * yi);
-        real_t const wi = ELEM_IM(w) - (a.re * yi + a.im * yr);
-        ELEM_RE(w) = wr; ELEM_IM(w) = wi;
-        acc[0] += double(wr) * wr + double(wi) * wi;
+template <typename T>
+__global__ __launch_bounds__(256) void k_even(T* __restrict__ un, T const* __restrict__ u, T const* __restrict__ v,
+                                              T* __restrict__ w, T const* __restrict__ Au, double const* __restrict__ S,
+                                              uint32_t ncol, SegArgs g) {
+    SEG_PROLOGUE
+    Cx<T> a[V];
+    coefs(a, S, ncol, q0, ALPHA_RE);
+    double acc[1][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T ur[V], ui[V], vr[V], vi[V], wr[V], wi[V], yr[V], yi[V];
+            ld(ur, u + OFF(0, i)); ld(ui, u + OFF(1, i));
+            ld(vr, v + OFF(0, i)); ld(vi, v + OFF(1, i));
+            ld(wr, w + OFF(0, i)); ld(wi, w + OFF(1, i));
+            ld(yr, Au + OFF(0, i)); ld(yi, Au + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                ur[t] = ur[t] - (a[t].re * vr[t] - a[t].im * vi[t]);
+                ui[t] = ui[t] - (a[t].re * vi[t] + a[t].im * vr[t]);
+                wr[t] = wr[t] - (a[t].re * yr[t] - a[t].im * yi[t]);
+                wi[t] = wi[t] - (a[t].re * yi[t] + a[t].im * yr[t]);
+                acc[0][t] += double(wr[t]) * wr[t] + double(wi[t]) * wi[t];
+            }
+            st(un + OFF(0, i), ur); st(un + OFF(1, i), ui);
+            st(w + OFF(0, i), wr); st(w + OFF(1, i), wi);
+        }
     }
-    store_partials<1>(acc, part, blk);
+    store_partials<T, 1>(acc, g.segpart, s);
 }
 
-// odd half-step, first part: w -= alpha Au; partials |w|^2 and dot(rs, w)
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_odd_w(real_t* __restrict__ w, real_t const* __restrict__ Au,
-                                               uint16_t const* __restrict__ colindx, double const* __restrict__ S,
-                                               uint32_t ncol, double* __restrict__ part, uint64_t seed) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    uint32_t const q = uint32_t(colindx[blk]) * 64 + j;
-    auto const a = coef<real_t>(S, ncol, q, ALPHA_RE);
-    double acc[3] = {0, 0, 0};
-    ELEMENT_LOOP {
-        int const i = 4 * k + (tid >> 6);
-        real_t const yr = ELEM_RE(Au), yi = ELEM_IM(Au);
-        real_t const wr = ELEM_RE(w) - (a.re * yr - a.im * yi);
-        real_t const wi = ELEM_IM(w) - (a.re * yi + a.im * yr);
-        ELEM_RE(w) = wr; ELEM_IM(w) = wi;
-        acc[0] += double(wr) * wr + double(wi) * wi;
-        double const sr = shadow_value(seed, elem_index(blk, 0, i, j)), si = shadow_value(seed, elem_index(blk, 1, i, j));
-        acc[1] += sr * wr + si * wi;
-        acc[2] += sr * wi - si * wr;
+// odd half-step: w -= alpha Au; partials |w|^2 and (r0*, w)
+template <typename T>
+__global__ __launch_bounds__(256) void k_odd(T* __restrict__ w, T const* __restrict__ Au, double const* __restrict__ S,
+                                             uint32_t ncol, SegArgs g, uint64_t seed) {
+    SEG_PROLOGUE
+    Cx<T> a[V];
+    coefs(a, S, ncol, q0, ALPHA_RE);
+    double acc[3][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T wr[V], wi[V], yr[V], yi[V];
+            ld(wr, w + OFF(0, i)); ld(wi, w + OFF(1, i));
+            ld(yr, Au + OFF(0, i)); ld(yi, Au + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                wr[t] = wr[t] - (a[t].re * yr[t] - a[t].im * yi[t]);
+                wi[t] = wi[t] - (a[t].re * yi[t] + a[t].im * yr[t]);
+                double const sr = shadow_value(seed, elem_index(blk, 0, i, j0 + t));
+                double const si = shadow_value(seed, elem_index(blk, 1, i, j0 + t));
+                double const xr = wr[t], xi = wi[t];
+                acc[0][t] += xr * xr + xi * xi;
+                acc[1][t] += sr * xr + si * xi;
+                acc[2][t] += sr * xi - si * xr;
+            }
+            st(w + OFF(0, i), wr); st(w + OFF(1, i), wi);
+        }
     }
-    store_partials<3>(acc, part, blk);
+    store_partials<T, 3>(acc, g.segpart, s);
 }
 
-// d = u + cd d; x += eta d
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_dx(real_t* __restrict__ d, real_t* __restrict__ x, real_t const* __restrict__ u,
-                                            uint16_t const* __restrict__ colindx, double const* __restrict__ S, uint32_t ncol) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    uint32_t const q = uint32_t(colindx[blk]) * 64 + j;
-    auto const c = coef<real_t>(S, ncol, q, CD_RE);
-    auto const e = coef<real_t>(S, ncol, q, ETA_RE);
-    ELEMENT_LOOP {
-        real_t const dr0 = ELEM_RE(d), di0 = ELEM_IM(d);
-        real_t const dr = ELEM_RE(u) + (c.re * dr0 - c.im * di0);
-        real_t const di = ELEM_IM(u) + (c.re * di0 + c.im * dr0);
-        ELEM_RE(d) = dr; ELEM_IM(d) = di;
-        ELEM_RE(x) += e.re * dr - e.im * di;
-        ELEM_IM(x) += e.re * di + e.im * dr;
+// both x updates of a step and the next u:
+//   d1 = u + cdE d; x += etaE d1; d2 = un + cdO d1; x += etaO d2; d = d2; u = w + beta un
+template <typename T>
+__global__ __launch_bounds__(256) void k_dx2(T* __restrict__ d, double* __restrict__ x, T* __restrict__ u,
+                                             T const* __restrict__ un, T const* __restrict__ w,
+                                             double const* __restrict__ S, uint32_t ncol, SegArgs g) {
+    SEG_PROLOGUE
+    Cx<T> ce[V], co[V], be[V];
+    Cx<double> ee[V], eo[V];
+    coefs(ce, S, ncol, q0, CDE_RE);
+    coefs(co, S, ncol, q0, CDO_RE);
+    coefs(be, S, ncol, q0, BETA_RE);
+    coefs(ee, S, ncol, q0, ETAE_RE);
+    coefs(eo, S, ncol, q0, ETAO_RE);
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T dr[V], di[V], ur[V], ui[V], nr[V], ni[V], wr[V], wi[V];
+            double xr[V], xi[V];
+            ld(dr, d + OFF(0, i)); ld(di, d + OFF(1, i));
+            ld(ur, u + OFF(0, i)); ld(ui, u + OFF(1, i));
+            ld(nr, un + OFF(0, i)); ld(ni, un + OFF(1, i));
+            ld(wr, w + OFF(0, i)); ld(wi, w + OFF(1, i));
+            ldd<V>(xr, x + OFF(0, i)); ldd<V>(xi, x + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                T const d1r = ur[t] + (ce[t].re * dr[t] - ce[t].im * di[t]);
+                T const d1i = ui[t] + (ce[t].re * di[t] + ce[t].im * dr[t]);
+                xr[t] += ee[t].re * double(d1r) - ee[t].im * double(d1i);
+                xi[t] += ee[t].re * double(d1i) + ee[t].im * double(d1r);
+                T const d2r = nr[t] + (co[t].re * d1r - co[t].im * d1i);
+                T const d2i = ni[t] + (co[t].re * d1i + co[t].im * d1r);
+                xr[t] += eo[t].re * double(d2r) - eo[t].im * double(d2i);
+                xi[t] += eo[t].re * double(d2i) + eo[t].im * double(d2r);
+                dr[t] = d2r; di[t] = d2i;
+                ur[t] = wr[t] + (be[t].re * nr[t] - be[t].im * ni[t]);
+                ui[t] = wi[t] + (be[t].re * ni[t] + be[t].im * nr[t]);
+            }
+            st(d + OFF(0, i), dr); st(d + OFF(1, i), di);
+            st(u + OFF(0, i), ur); st(u + OFF(1, i), ui);
+            std_<V>(x + OFF(0, i), xr); std_<V>(x + OFF(1, i), xi);
+        }
     }
 }
 
-// un = w + beta u
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_unext(real_t* __restrict__ un, real_t const* __restrict__ w, real_t const* __restrict__ u,
-                                               uint16_t const* __restrict__ colindx, double const* __restrict__ S, uint32_t ncol) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    uint32_t const q = uint32_t(colindx[blk]) * 64 + j;
-    auto const b = coef<real_t>(S, ncol, q, BETA_RE);
-    ELEMENT_LOOP {
-        real_t const ur = ELEM_RE(u), ui = ELEM_IM(u);
-        ELEM_RE(un) = ELEM_RE(w) + (b.re * ur - b.im * ui);
-        ELEM_IM(un) = ELEM_IM(w) + (b.re * ui + b.im * ur);
+// v = Au + beta (Aun_prev + beta v); partial (r0*, v)
+template <typename T>
+__global__ __launch_bounds__(256) void k_vnext(T* __restrict__ v, T const* __restrict__ Au, T const* __restrict__ Aodd,
+                                               double const* __restrict__ S, uint32_t ncol, SegArgs g, uint64_t seed) {
+    SEG_PROLOGUE
+    Cx<T> be[V];
+    coefs(be, S, ncol, q0, BETA_RE);
+    double acc[2][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T vr[V], vi[V], ar[V], ai[V], or_[V], oi[V];
+            ld(vr, v + OFF(0, i)); ld(vi, v + OFF(1, i));
+            ld(ar, Au + OFF(0, i)); ld(ai, Au + OFF(1, i));
+            ld(or_, Aodd + OFF(0, i)); ld(oi, Aodd + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                T const tr = or_[t] + (be[t].re * vr[t] - be[t].im * vi[t]);
+                T const ti = oi[t] + (be[t].re * vi[t] + be[t].im * vr[t]);
+                vr[t] = ar[t] + (be[t].re * tr - be[t].im * ti);
+                vi[t] = ai[t] + (be[t].re * ti + be[t].im * tr);
+                double const sr = shadow_value(seed, elem_index(blk, 0, i, j0 + t));
+                double const si = shadow_value(seed, elem_index(blk, 1, i, j0 + t));
+                acc[0][t] += sr * vr[t] + si * vi[t];
+                acc[1][t] += sr * vi[t] - si * vr[t];
+            }
+            st(v + OFF(0, i), vr); st(v + OFF(1, i), vi);
+        }
     }
+    store_partials<T, 2>(acc, g.segpart, s);
 }
 
-// v = Aun + beta (Au + beta v)
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_vnext(real_t* __restrict__ v, real_t const* __restrict__ Aun, real_t const* __restrict__ Au,
-                                               uint16_t const* __restrict__ colindx, double const* __restrict__ S, uint32_t ncol) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    uint32_t const q = uint32_t(colindx[blk]) * 64 + j;
-    auto const b = coef<real_t>(S, ncol, q, BETA_RE);
-    ELEMENT_LOOP {
-        real_t const vr = ELEM_RE(v), vi = ELEM_IM(v);
-        real_t const tr = ELEM_RE(Au) + (b.re * vr - b.im * vi);
-        real_t const ti = ELEM_IM(Au) + (b.re * vi + b.im * vr);
-        ELEM_RE(v) = ELEM_RE(Aun) + (b.re * tr - b.im * ti);
-        ELEM_IM(v) = ELEM_IM(Aun) + (b.re * ti + b.im * tr);
-    }
-}
-
-// r = b - y (FP64); partial |r|^2; optionally rhs32 = float(r)
-template <typename out_t>
+// r = b - y (FP64, b = unit columns on the diagonal blocks); partial |r|^2; optionally r_out = r
+template <typename T>
 __global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, uint8_t const* __restrict__ diag,
-                                                  double* __restrict__ part, out_t* __restrict__ r_out) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x, j = tid & 63;
-    bool const dg = diag[blk];
-    double acc[1] = {0};
-    ELEMENT_LOOP {
-        int const i = 4 * k + (tid >> 6);
-        double const rr = ((dg && i == j) ? 1.0 : 0.0) - ELEM_RE(y);
-        double const ri = -ELEM_IM(y);
-        acc[0] += rr * rr + ri * ri;
-        if (r_out) { ELEM_RE(r_out) = out_t(rr); ELEM_IM(r_out) = out_t(ri); }
+                                                  T* __restrict__ r_out, SegArgs g) {
+    SEG_PROLOGUE
+    double acc[1][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+        bool const dg = diag[blk];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            double yr[V], yi[V];
+            ldd<V>(yr, y + OFF(0, i)); ldd<V>(yi, y + OFF(1, i));
+            T orr[V], oi[V];
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                double const rr = ((dg && i == j0 + t) ? 1.0 : 0.0) - yr[t], ri = -yi[t];
+                acc[0][t] += rr * rr + ri * ri;
+                orr[t] = T(rr); oi[t] = T(ri);
+            }
+            if (r_out) { st(r_out + OFF(0, i), orr); st(r_out + OFF(1, i), oi); }
+        }
     }
-    store_partials<1>(acc, part, blk);
-}
-
-// x64 += delta
-template <typename real_t>
-__global__ __launch_bounds__(256) void k_accumulate(double* __restrict__ x64, real_t const* __restrict__ delta) {
-    uint64_t const blk = blockIdx.x;
-    int const tid = threadIdx.x;
-    for (int k = 0; k < 32; ++k) {
-        uint64_t const e = blk * NBLK + k * 256 + tid;
-        x64[e] += double(delta[e]);
-    }
+    store_partials<T, 1>(acc, g.segpart, s);
 }
 
 // ------------------------------------------------------------------------------------------------
-// per-column scalar updates
+// per-column scalar updates; partials are folded over the column group's segments in a fixed order
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ double& F(double* S, uint32_t ncol, int f, uint32_t q) { return S[uint64_t(f) * ncol + q]; }
 
-__device__ __forceinline__ void freeze(double* S, uint32_t ncol, uint32_t q) {
-    F(S, ncol, ALPHA_RE, q) = 0; F(S, ncol, ALPHA_IM, q) = 0;
-    F(S, ncol, ETA_RE, q) = 0; F(S, ncol, ETA_IM, q) = 0;
-    F(S, ncol, CD_RE, q) = 0; F(S, ncol, CD_IM, q) = 0;
-    F(S, ncol, BETA_RE, q) = 0; F(S, ncol, BETA_IM, q) = 0;
+__device__ __forceinline__ double fold(double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
+                                       uint32_t q, int c) {
+    uint32_t const col = q >> 6, j = q & 63;
+    double s = 0;
+    for (uint32_t k = col_seg[col]; k < col_seg[col + 1]; ++k) s += segpart[(uint64_t(k) * 3 + c) * 64 + j];
+    return s;
 }
 
-// after k_init: red = |rhs|^2, dot(rs, rhs)
-__global__ void k_scalar_init(double* __restrict__ S, double const* __restrict__ red, uint32_t ncol,
-                              double tol_rel, double const* __restrict__ tol_abs) {
+__device__ __forceinline__ void freeze_step(double* S, uint32_t ncol, uint32_t q) {
+    for (int f = ALPHA_RE; f <= ALPHA_IM; ++f) F(S, ncol, f, q) = 0;
+    for (int f = ETAE_RE; f <= BETA_IM; ++f) F(S, ncol, f, q) = 0;
+}
+
+// after k_init: |b|^2, (r0*, b); columns of inactive groups start as converged
+__global__ void k_scalar_init(double* __restrict__ S, double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
+                              uint8_t const* __restrict__ col_active, uint32_t ncol, double tol_rel,
+                              double const* __restrict__ tol_abs) {
     uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ncol) return;
-    double const bn = sqrt(red[q]);
+    if (!col_active[q >> 6]) {
+        F(S, ncol, STATUS, q) = CONVERGED;
+        F(S, ncol, MSTEP, q) = 0;
+        freeze_step(S, ncol, q);
+        return;
+    }
+    double const bn = sqrt(fold(segpart, col_seg, q, 0));
     F(S, ncol, BNORM, q) = bn;
     F(S, ncol, TAU, q) = bn;
     F(S, ncol, THETA, q) = 0;
-    F(S, ncol, RHO_RE, q) = red[ncol + q];
-    F(S, ncol, RHO_IM, q) = red[2 * ncol + q];
+    F(S, ncol, RHO_RE, q) = fold(segpart, col_seg, q, 1);
+    F(S, ncol, RHO_IM, q) = fold(segpart, col_seg, q, 2);
     double const tol = fmax(tol_rel * bn, tol_abs ? tol_abs[q] : 0.0);
     F(S, ncol, TOL, q) = tol;
-    freeze(S, ncol, q);
+    F(S, ncol, MSTEP, q) = 0;
+    freeze_step(S, ncol, q);
     // nothing to do for a column whose right-hand side already meets the tolerance
     F(S, ncol, STATUS, q) = (bn <= tol) ? CONVERGED : ACTIVE;
 }
 
-// even half-step: alpha = rho / dot(rs, v)
-__global__ void k_scalar_alpha(double* __restrict__ S, double const* __restrict__ red, uint32_t ncol) {
+// start of a step: alpha = rho / (r0*, v); a column that is no longer active gets zero coefficients
+__global__ void k_scalar_alpha(double* __restrict__ S, double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
+                               uint32_t ncol) {
     uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ncol) return;
-    if (F(S, ncol, STATUS, q) != ACTIVE) { freeze(S, ncol, q); return; }
-    double const sr = red[q], si = red[ncol + q];
+    if (F(S, ncol, STATUS, q) != ACTIVE) { freeze_step(S, ncol, q); return; }
+    double const sr = fold(segpart, col_seg, q, 0), si = fold(segpart, col_seg, q, 1);
     double const den = sr * sr + si * si;
-    if (!(den > 0)) { F(S, ncol, STATUS, q) = BREAKDOWN; freeze(S, ncol, q); return; }
+    if (!(den > 0)) { F(S, ncol, STATUS, q) = BREAKDOWN; freeze_step(S, ncol, q); return; }
     double const rr = F(S, ncol, RHO_RE, q), ri = F(S, ncol, RHO_IM, q);
     F(S, ncol, ALPHA_RE, q) = (rr * sr + ri * si) / den;
     F(S, ncol, ALPHA_IM, q) = (ri * sr - rr * si) / den;
 }
 
-// after the w update: theta, c, tau, eta, the d coefficient, convergence; for odd steps also rho, beta
-__global__ void k_scalar_theta(double* __restrict__ S, double const* __restrict__ red, uint32_t ncol, int m, int odd) {
+// after a w update: theta, c, tau, eta, the d coefficient, convergence; on odd half-steps also rho, beta.
+// Even half-steps write the E fields, odd ones the O fields (both are used by k_dx2).
+template <int ODD>
+__global__ void k_scalar_theta(double* __restrict__ S, double const* __restrict__ segpart,
+                               uint32_t const* __restrict__ col_seg, uint32_t ncol) {
     uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ncol) return;
-    if (F(S, ncol, STATUS, q) != ACTIVE) { freeze(S, ncol, q); return; }
+    int const FE = ODD ? ETAO_RE : ETAE_RE, FC = ODD ? CDO_RE : CDE_RE, FP = ODD ? ETAE_RE : ETAO_RE;
+    if (F(S, ncol, STATUS, q) != ACTIVE) {
+        F(S, ncol, FE, q) = 0; F(S, ncol, FE + 1, q) = 0;
+        F(S, ncol, FC, q) = 0; F(S, ncol, FC + 1, q) = 0;
+        if (ODD) { F(S, ncol, BETA_RE, q) = 0; F(S, ncol, BETA_IM, q) = 0; }
+        return;
+    }
+    double const m = F(S, ncol, MSTEP, q);
     double const ar = F(S, ncol, ALPHA_RE, q), ai = F(S, ncol, ALPHA_IM, q);
-    double const th = F(S, ncol, THETA, q), er = F(S, ncol, ETA_RE, q), ei = F(S, ncol, ETA_IM, q);
+    double const th = F(S, ncol, THETA, q), er = F(S, ncol, FP, q), ei = F(S, ncol, FP + 1, q);
     // cd = theta^2 eta / alpha with the previous theta, eta
     double const a2 = ar * ar + ai * ai;
     double cr = 0, ci = 0;
@@ -358,17 +472,18 @@ __global__ void k_scalar_theta(double* __restrict__ S, double const* __restrict_
         ci = (ti * ar - tr * ai) / a2;
     }
     double const tau = F(S, ncol, TAU, q);
-    double const thn = sqrt(red[q]) / tau;
+    double const thn = sqrt(fold(segpart, col_seg, q, 0)) / tau;
     double const c = 1.0 / sqrt(1.0 + thn * thn);
     double const taun = tau * thn * c;
-    F(S, ncol, CD_RE, q) = cr; F(S, ncol, CD_IM, q) = ci;
+    F(S, ncol, FC, q) = cr; F(S, ncol, FC + 1, q) = ci;
     F(S, ncol, THETA, q) = thn;
     F(S, ncol, TAU, q) = taun;
-    F(S, ncol, ETA_RE, q) = c * c * ar;
-    F(S, ncol, ETA_IM, q) = c * c * ai;
-    if (taun * sqrt(double(m + 2)) <= F(S, ncol, TOL, q)) F(S, ncol, STATUS, q) = CONVERGED;  // x still gets this step
-    if (odd) {
-        double const rnr = red[ncol + q], rni = red[2 * ncol + q];
+    F(S, ncol, FE, q) = c * c * ar;
+    F(S, ncol, FE + 1, q) = c * c * ai;
+    F(S, ncol, MSTEP, q) = m + 1;
+    if (taun * sqrt(m + 2) <= F(S, ncol, TOL, q)) F(S, ncol, STATUS, q) = CONVERGED;  // x still gets this step
+    if (ODD) {
+        double const rnr = fold(segpart, col_seg, q, 1), rni = fold(segpart, col_seg, q, 2);
         double const rr = F(S, ncol, RHO_RE, q), ri = F(S, ncol, RHO_IM, q);
         double const den = rr * rr + ri * ri;
         if (!(den > 0) || !(rnr * rnr + rni * rni > 0)) {
@@ -380,12 +495,25 @@ __global__ void k_scalar_theta(double* __restrict__ S, double const* __restrict_
         }
         F(S, ncol, RHO_RE, q) = rnr; F(S, ncol, RHO_IM, q) = rni;
     }
-    if (!(taun == taun)) { F(S, ncol, STATUS, q) = BREAKDOWN; freeze(S, ncol, q); }
+    if (!(taun == taun)) {
+        F(S, ncol, STATUS, q) = BREAKDOWN;
+        F(S, ncol, FE, q) = 0; F(S, ncol, FE + 1, q) = 0;
+        F(S, ncol, FC, q) = 0; F(S, ncol, FC + 1, q) = 0;
+    }
 }
 
-__global__ void k_sqrt_copy(double* __restrict__ out, double const* __restrict__ red, uint32_t n) {
+// residual norms of the active column groups (others keep their value)
+__global__ void k_col_norm(double* __restrict__ res, double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
+                           uint8_t const* __restrict__ col_active, uint32_t ncol) {
     uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) out[q] = sqrt(red[q]);
+    if (q >= ncol || !col_active[q >> 6]) return;
+    res[q] = sqrt(fold(segpart, col_seg, q, 0));
+}
+
+// x = 0 over whole blocks (a fill kernel rather than a memset, so that it may sit in a captured graph)
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ x, uint64_t n) {
+    uint64_t const i = (uint64_t(blockIdx.x) * 256 + threadIdx.x) * 2;
+    if (i < n) { d2 z; z[0] = 0; z[1] = 0; *reinterpret_cast<d2*>(x + i) = z; }
 }
 
 // rho[c][i] = G[subset[c]][*][i][i] / pi
@@ -404,8 +532,45 @@ __global__ void k_diag(double* __restrict__ out, double const* __restrict__ X, u
 // host side
 // ------------------------------------------------------------------------------------------------
 struct SolverState {
-    SolverWs ws;
+    bool ready = false;
+    uint64_t nnzb = 0;
+    uint32_t nCols = 0, ncol = 0;
+    hipStream_t stream = nullptr;
+    // static structure
+    std::vector<uint32_t> h_cstart, h_cb;   // blocks of column group c: h_cb[h_cstart[c] .. h_cstart[c+1])
+    uint32_t* col_blocks = nullptr;
+    uint8_t* diag = nullptr;
     uint32_t* subset = nullptr;
+    // segments of <= SEG blocks of one column group, numbered once for the whole solve
+    uint32_t nseg_all = 0;
+    uint32_t* seg = nullptr;                // [nseg_all][3]
+    uint32_t* col_seg = nullptr;            // [nCols+1] segment range of each column group
+    std::vector<uint32_t> h_col_seg;
+    // active set
+    std::vector<uint8_t> active;            // [nCols]
+    uint32_t nseg = 0;                      // active segments
+    uint32_t* act_seg = nullptr;            // [nseg_all] global numbers of the active segments
+    uint32_t* act_order = nullptr;          // [nnzb] blocks of active groups in the action's order
+    uint64_t n_act_blocks = 0;
+    uint8_t* col_active = nullptr;          // [nCols]
+    std::vector<uint32_t> h_act_seg, h_act;
+    // reductions and scalars
+    double* segpart = nullptr;              // [nseg_all][3][64]
+    double* S = nullptr;                    // [NFIELDS][ncol]
+    double* S_host = nullptr;               // pinned copy of the status field
+    double* res = nullptr;                  // [ncol] true residual norms
+    double* tol_abs = nullptr;              // [ncol]
+    // vectors
+    double* x64 = nullptr;                  // the FP64 solution
+    double* rhs64 = nullptr;                // FP64 residual (refinement right-hand side)
+    float* rhs32 = nullptr;
+    double* work64 = nullptr;               // 7 FP64 work vectors, contiguous
+    float* work32 = nullptr;                // 7 FP32 work vectors, contiguous
+    // captured step
+    hipGraphExec_t graph = nullptr;
+    int graph_fp = 0;
+    uint64_t graph_seed = 0;
+    std::vector<void*> allocations;
 };
 
 namespace {
@@ -421,158 +586,222 @@ SolverState* state_of(gfa_plan* p, bool create) {
 }
 
 template <typename T>
-int walloc(SolverWs& ws, T*& ptr, size_t n, std::string& err) {
+int walloc(SolverState& st, T*& ptr, size_t n, std::string& err) {
     void* q = nullptr;
     HIP_TRY(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
-    ws.allocations.push_back(q);
+    st.allocations.push_back(q);
     ptr = (T*)q;
     return 0;
 }
 
+void drop_graph(SolverState& st) {
+    if (st.graph) (void)hipGraphExecDestroy(st.graph);
+    st.graph = nullptr;
+}
+
 int setup(gfa_plan* plan, SolverState& st, std::string& err) {
-    auto& ws = st.ws;
-    if (ws.ready) return 0;
+    if (st.ready) return 0;
     Plan const& h = capi_host(plan);
-    ws.nnzb = h.nnzb;
-    ws.ncol = 64 * h.nCols;
-    // blocks grouped by column, in ascending block order, and segments of SEG blocks
-    std::vector<uint32_t> count(h.nCols + 1, 0), cb(h.nnzb), segs, col_seg(1, 0);
-    for (uint64_t b = 0; b < h.nnzb; ++b) ++count[h.colindx[b] + 1];
-    for (uint32_t c = 0; c < h.nCols; ++c) count[c + 1] += count[c];
-    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
-    for (uint64_t b = 0; b < h.nnzb; ++b) cb[fill[h.colindx[b]]++] = uint32_t(b);
-    for (uint32_t c = 0; c < h.nCols; ++c) {
-        for (uint32_t b = count[c]; b < count[c + 1]; b += SEG) {
-            segs.push_back(c); segs.push_back(b); segs.push_back(std::min(count[c + 1], b + SEG));
-        }
-        col_seg.push_back(uint32_t(segs.size() / 3));
+    st.nnzb = h.nnzb;
+    st.nCols = h.nCols;
+    st.ncol = 64 * h.nCols;
+    HIP_TRY(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+    // blocks grouped by column group, ascending
+    st.h_cstart.assign(h.nCols + 1, 0);
+    st.h_cb.resize(h.nnzb);
+    for (uint64_t b = 0; b < h.nnzb; ++b) ++st.h_cstart[h.colindx[b] + 1];
+    for (uint32_t c = 0; c < h.nCols; ++c) st.h_cstart[c + 1] += st.h_cstart[c];
+    {
+        std::vector<uint32_t> fill(st.h_cstart.begin(), st.h_cstart.end() - 1);
+        for (uint64_t b = 0; b < h.nnzb; ++b) st.h_cb[fill[h.colindx[b]]++] = uint32_t(b);
     }
-    ws.nseg = uint32_t(segs.size() / 3);
+    std::vector<uint32_t> segs;
+    st.h_col_seg.assign(1, 0);
+    for (uint32_t c = 0; c < h.nCols; ++c) {
+        for (uint32_t b = st.h_cstart[c]; b < st.h_cstart[c + 1]; b += SEG) {
+            segs.push_back(c); segs.push_back(b); segs.push_back(std::min(st.h_cstart[c + 1], b + SEG));
+        }
+        st.h_col_seg.push_back(uint32_t(segs.size() / 3));
+    }
+    st.nseg_all = uint32_t(segs.size() / 3);
     std::vector<uint8_t> diag(h.nnzb, 0);
     for (uint32_t c = 0; c < h.nCols; ++c) diag[h.subset[c]] = 1;
     int s;
-    if ((s = walloc(ws, ws.col_blocks, cb.size(), err))) return s;
-    if ((s = walloc(ws, ws.seg, segs.size(), err))) return s;
-    if ((s = walloc(ws, ws.col_seg, col_seg.size(), err))) return s;
-    if ((s = walloc(ws, ws.diag, diag.size(), err))) return s;
-    if ((s = walloc(ws, st.subset, h.subset.size(), err))) return s;
-    HIP_TRY(hipMemcpy(ws.col_blocks, cb.data(), cb.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(ws.seg, segs.data(), segs.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(ws.col_seg, col_seg.data(), col_seg.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(ws.diag, diag.data(), diag.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(st.subset, h.subset.data(), h.subset.size() * 4, hipMemcpyHostToDevice));
-    if ((s = walloc(ws, ws.part, size_t(h.nnzb) * 3 * 64, err))) return s;
-    if ((s = walloc(ws, ws.segpart, size_t(ws.nseg) * 3 * 64, err))) return s;
-    if ((s = walloc(ws, ws.red, size_t(3) * ws.ncol, err))) return s;
-    if ((s = walloc(ws, ws.S, size_t(NFIELDS) * ws.ncol, err))) return s;
-    if ((s = walloc(ws, ws.res, ws.ncol, err))) return s;
-    if ((s = walloc(ws, ws.tol_abs, ws.ncol, err))) return s;
-    HIP_TRY(hipHostMalloc((void**)&ws.S_host, size_t(ws.ncol) * sizeof(double), hipHostMallocDefault));
-    ws.ready = true;
+    if ((s = walloc(st, st.col_blocks, h.nnzb, err))) return s;
+    if ((s = walloc(st, st.diag, h.nnzb, err))) return s;
+    if ((s = walloc(st, st.subset, h.nCols, err))) return s;
+    if ((s = walloc(st, st.seg, size_t(st.nseg_all) * 3, err))) return s;
+    if ((s = walloc(st, st.col_seg, size_t(h.nCols) + 1, err))) return s;
+    if ((s = walloc(st, st.act_seg, st.nseg_all, err))) return s;
+    if ((s = walloc(st, st.act_order, h.nnzb, err))) return s;
+    if ((s = walloc(st, st.col_active, h.nCols, err))) return s;
+    HIP_TRY(hipMemcpy(st.col_blocks, st.h_cb.data(), h.nnzb * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(st.diag, diag.data(), h.nnzb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(st.subset, h.subset.data(), size_t(h.nCols) * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(st.seg, segs.data(), segs.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(st.col_seg, st.h_col_seg.data(), st.h_col_seg.size() * 4, hipMemcpyHostToDevice));
+    if ((s = walloc(st, st.segpart, size_t(st.nseg_all) * 3 * 64, err))) return s;
+    if ((s = walloc(st, st.S, size_t(NFIELDS) * st.ncol, err))) return s;
+    if ((s = walloc(st, st.res, st.ncol, err))) return s;
+    if ((s = walloc(st, st.tol_abs, st.ncol, err))) return s;
+    HIP_TRY(hipHostMalloc((void**)&st.S_host, size_t(st.ncol) * sizeof(double), hipHostMallocDefault));
+    if ((s = walloc(st, st.x64, h.nnzb * NBLK, err))) return s;
+    st.ready = true;
     return 0;
 }
 
-template <typename real_t>
-void** vecs(SolverWs& ws);
-template <> void** vecs<double>(SolverWs& ws) { return ws.v64; }
-template <> void** vecs<float>(SolverWs& ws) { return ws.v32; }
+// the active segments, the action's block list and the column flags for the active column groups;
+// segment numbers stay fixed, so partial sums written before a retirement keep their meaning
+int set_active(gfa_plan* plan, SolverState& st, std::vector<uint8_t> const& active, std::string& err) {
+    st.active = active;
+    st.h_act_seg.clear();
+    for (uint32_t c = 0; c < st.nCols; ++c)
+        if (active[c])
+            for (uint32_t k = st.h_col_seg[c]; k < st.h_col_seg[c + 1]; ++k) st.h_act_seg.push_back(k);
+    st.nseg = uint32_t(st.h_act_seg.size());
+    DevicePlan const& dev = capi_dev(plan);
+    Plan const& h = capi_host(plan);
+    st.h_act.clear();
+    for (uint32_t b : dev.h_order) if (active[h.colindx[b]]) st.h_act.push_back(b);
+    st.n_act_blocks = st.h_act.size();
+    if (st.nseg) HIP_TRY(hipMemcpyAsync(st.act_seg, st.h_act_seg.data(), st.nseg * 4, hipMemcpyHostToDevice, st.stream));
+    if (st.n_act_blocks)
+        HIP_TRY(hipMemcpyAsync(st.act_order, st.h_act.data(), st.n_act_blocks * 4, hipMemcpyHostToDevice, st.stream));
+    HIP_TRY(hipMemcpyAsync(st.col_active, active.data(), st.nCols, hipMemcpyHostToDevice, st.stream));
+    HIP_TRY(hipStreamSynchronize(st.stream));  // the host vectors may change before the copies ran otherwise
+    drop_graph(st);
+    return 0;
+}
 
-// allocate vectors lo..hi-1 of one precision (each nnzb blocks)
-template <typename real_t>
-int ensure_vectors(SolverWs& ws, int lo, int hi, std::string& err) {
-    void** v = vecs<real_t>(ws);
-    for (int k = lo; k < hi; ++k) {
-        if (v[k]) continue;
-        real_t* p = nullptr;
-        if (int s = walloc(ws, p, size_t(ws.nnzb) * NBLK, err)) return s;
-        v[k] = p;
+// the action restricted to the active set while it is in scope
+struct ActiveScope {
+    DevicePlan& d;
+    ActiveScope(DevicePlan& d_, SolverState const& st) : d(d_) {
+        d.active_order = st.act_order; d.n_active = st.n_act_blocks; d.col_active = st.col_active;
     }
-    return 0;
-}
+    ~ActiveScope() { d.active_order = nullptr; d.n_active = 0; d.col_active = nullptr; }
+};
 
-int reduce(SolverWs& ws, int nc, hipStream_t stream, std::string& err) {
-    hipLaunchKernelGGL(k_seg_reduce, dim3(ws.nseg), dim3(64), 0, stream, ws.segpart, ws.part, ws.seg, ws.col_blocks, nc);
-    hipLaunchKernelGGL(k_col_reduce, dim3(ws.ncol / 64), dim3(64), 0, stream, ws.red, ws.segpart, ws.col_seg, ws.ncol, nc);
-    HIP_TRY(hipGetLastError());
+template <typename T> T* work(SolverState& st, int k);
+template <> double* work<double>(SolverState& st, int k) { return st.work64 + size_t(k) * st.nnzb * NBLK; }
+template <> float* work<float>(SolverState& st, int k) { return st.work32 + size_t(k) * st.nnzb * NBLK; }
+
+template <typename T>
+int ensure_work(SolverState& st, std::string& err) {
+    if (sizeof(T) == 8 && !st.work64) return walloc(st, st.work64, 7 * st.nnzb * NBLK, err);
+    if (sizeof(T) == 4 && !st.work32) return walloc(st, st.work32, 7 * st.nnzb * NBLK, err);
     return 0;
 }
 
 struct TfqmrResult { int halfsteps = 0; int actions = 0; };
 
-// TFQMR on real_t vectors for the right-hand side rhs (nullptr: unit columns); solution into x
-template <typename real_t>
-int tfqmr(gfa_plan* plan, SolverState& st, real_t* x, real_t const* rhs, double tol_rel, double const* tol_abs,
-          int maxit, int check_every, uint64_t seed, hipStream_t stream, TfqmrResult& res, std::string& err) {
-    auto& ws = st.ws;
-    if (int s = ensure_vectors<real_t>(ws, 1, 8, err)) return s;
-    void** V = vecs<real_t>(ws);
-    // V[0] x is the caller's x; buffers: w, u0, u1, v, d, Aodd, Aeven
-    real_t* w = (real_t*)V[1];
-    real_t* uu[2] = {(real_t*)V[2], (real_t*)V[3]};
-    real_t* v = (real_t*)V[4];
-    real_t* d = (real_t*)V[5];
-    real_t* Aodd = (real_t*)V[6];
-    real_t* Aeven = (real_t*)V[7];
-    int const fp = sizeof(real_t) == 8 ? 64 : 32;
-    uint32_t const ncol = ws.ncol;
-    dim3 const gb(uint32_t(ws.nnzb)), tb(256), gs((ncol + 255) / 256), ts(256);
-    auto const& dev = capi_dev(plan);
-    uint16_t const* colindx = dev.colindx;
-    int cur = 0;
+// work vectors of one TFQMR run
+template <typename T> struct Vecs { T *w, *u, *un, *v, *d, *Aodd, *Aeven; };
 
-    hipLaunchKernelGGL((k_init<real_t>), gb, tb, 0, stream, x, d, w, uu[0], rhs, ws.diag, ws.part, seed);
-    if (int s = reduce(ws, 3, stream, err)) return s;
-    hipLaunchKernelGGL(k_scalar_init, gs, ts, 0, stream, ws.S, ws.red, ncol, tol_rel, tol_abs);
-    if (int s = capi_action(plan, fp, v, uu[0], stream, err)) return s;
+// one full step (even + odd half-step); all arguments are invariant between steps
+template <typename T>
+int enqueue_step(gfa_plan* plan, SolverState& st, Vecs<T> const& V, uint64_t seed, hipStream_t s, std::string& err) {
+    int const fp = sizeof(T) == 8 ? 64 : 32;
+    SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+    dim3 const gb(st.nseg), tb(256), gs((st.ncol + 255) / 256), ts(256);
+    hipLaunchKernelGGL(k_scalar_alpha, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.ncol);
+    hipLaunchKernelGGL((k_even<T>), gb, tb, 0, s, V.un, V.u, V.v, V.w, V.Aeven, st.S, st.ncol, g);
+    hipLaunchKernelGGL((k_scalar_theta<0>), gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.ncol);
+    if (int r = capi_action(plan, fp, V.Aodd, V.un, s, err)) return r;
+    hipLaunchKernelGGL((k_odd<T>), gb, tb, 0, s, V.w, V.Aodd, st.S, st.ncol, g, seed);
+    hipLaunchKernelGGL((k_scalar_theta<1>), gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.ncol);
+    hipLaunchKernelGGL((k_dx2<T>), gb, tb, 0, s, V.d, st.x64, V.u, V.un, V.w, st.S, st.ncol, g);
+    if (int r = capi_action(plan, fp, V.Aeven, V.u, s, err)) return r;
+    hipLaunchKernelGGL((k_vnext<T>), gb, tb, 0, s, V.v, V.Aeven, V.Aodd, st.S, st.ncol, g, seed);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// TFQMR on T vectors for the right-hand side rhs (nullptr: unit columns), accumulating into st.x64.
+// Column groups whose columns all stopped are retired from the active set as the solve goes.
+template <typename T>
+int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
+          int check_every, uint64_t seed, bool eager, TfqmrResult& res, std::string& err) {
+    if (int s = ensure_work<T>(st, err)) return s;
+    Vecs<T> V{work<T>(st, 0), work<T>(st, 1), work<T>(st, 2), work<T>(st, 3), work<T>(st, 4), work<T>(st, 5), work<T>(st, 6)};
+    int const fp = sizeof(T) == 8 ? 64 : 32;
+    hipStream_t const s = st.stream;
+    SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+    dim3 const gs((st.ncol + 255) / 256), ts(256);
+    DevicePlan& dev = capi_dev(plan);
+    ActiveScope scope(dev, st);
+    if (st.nseg == 0) return 0;
+
+    hipLaunchKernelGGL((k_init<T>), dim3(st.nseg), dim3(256), 0, s, V.d, V.w, V.u, rhs, st.diag, g, seed);
+    hipLaunchKernelGGL(k_scalar_init, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol, tol_rel, tol_abs);
+    // v = A u0 = Au_even of the first step
+    if (int r = capi_action(plan, fp, V.v, V.u, s, err)) return r;
     res.actions++;
-    real_t* Au_even = v;  // A u_0 = v_0
-    for (int m = 0; m < maxit; ++m) {
-        if ((m & 1) == 0) {
-            hipLaunchKernelGGL((k_dot_shadow<real_t>), gb, tb, 0, stream, v, ws.part, seed);
-            if (int s = reduce(ws, 2, stream, err)) return s;
-            hipLaunchKernelGGL(k_scalar_alpha, gs, ts, 0, stream, ws.S, ws.red, ncol);
-            hipLaunchKernelGGL((k_even<real_t>), gb, tb, 0, stream, uu[1 - cur], uu[cur], v, w, Au_even, colindx, ws.S, ncol, ws.part);
-            if (int s = reduce(ws, 1, stream, err)) return s;
-            hipLaunchKernelGGL(k_scalar_theta, gs, ts, 0, stream, ws.S, ws.red, ncol, m, 0);
-            hipLaunchKernelGGL((k_dx<real_t>), gb, tb, 0, stream, d, x, uu[cur], colindx, ws.S, ncol);
-            cur = 1 - cur;
+    HIP_TRY(hipMemcpyAsync(V.Aeven, V.v, st.nnzb * NBLK * sizeof(T), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL((k_dot_shadow<T>), dim3(st.nseg), dim3(256), 0, s, V.v, g, seed);
+    HIP_TRY(hipGetLastError());
+
+    int const steps_per_check = std::max(1, check_every / 2);
+    int m = 0;
+    while (m < maxit) {
+        int const nsteps = std::min(steps_per_check, (maxit - m + 1) / 2);
+        if (eager) {
+            for (int k = 0; k < nsteps; ++k)
+                if (int r = enqueue_step<T>(plan, st, V, seed, s, err)) return r;
         } else {
-            if (int s = capi_action(plan, fp, Aodd, uu[cur], stream, err)) return s;
-            res.actions++;
-            hipLaunchKernelGGL((k_odd_w<real_t>), gb, tb, 0, stream, w, Aodd, colindx, ws.S, ncol, ws.part, seed);
-            if (int s = reduce(ws, 3, stream, err)) return s;
-            hipLaunchKernelGGL(k_scalar_theta, gs, ts, 0, stream, ws.S, ws.red, ncol, m, 1);
-            hipLaunchKernelGGL((k_dx<real_t>), gb, tb, 0, stream, d, x, uu[cur], colindx, ws.S, ncol);
-            hipLaunchKernelGGL((k_unext<real_t>), gb, tb, 0, stream, uu[1 - cur], w, uu[cur], colindx, ws.S, ncol);
-            if (int s = capi_action(plan, fp, Aeven, uu[1 - cur], stream, err)) return s;
-            res.actions++;
-            hipLaunchKernelGGL((k_vnext<real_t>), gb, tb, 0, stream, v, Aeven, Aodd, colindx, ws.S, ncol);
-            Au_even = Aeven;
-            cur = 1 - cur;
+            if (!st.graph || st.graph_fp != fp || st.graph_seed != seed) {
+                drop_graph(st);
+                hipGraph_t gr = nullptr;
+                HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int const r = enqueue_step<T>(plan, st, V, seed, s, err);
+                hipError_t const e = hipStreamEndCapture(s, &gr);
+                if (r) { if (gr) (void)hipGraphDestroy(gr); return r; }
+                if (e != hipSuccess) { err = std::string("graph capture: ") + hipGetErrorString(e); return GFA_ERR_DEVICE; }
+                hipError_t const e2 = hipGraphInstantiate(&st.graph, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (e2 != hipSuccess) { st.graph = nullptr; err = std::string("graph instantiate: ") + hipGetErrorString(e2); return GFA_ERR_DEVICE; }
+                st.graph_fp = fp;
+                st.graph_seed = seed;
+            }
+            for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(st.graph, s));
         }
-        HIP_TRY(hipGetLastError());
-        res.halfsteps = m + 1;
-        if ((m + 1) % check_every == 0 || m + 1 == maxit) {
-            HIP_TRY(hipMemcpyAsync(ws.S_host, ws.S + size_t(STATUS) * ncol, ncol * sizeof(double), hipMemcpyDeviceToHost, stream));
-            HIP_TRY(hipStreamSynchronize(stream));
-            bool any = false;
-            for (uint32_t q = 0; q < ncol && !any; ++q) any = (ws.S_host[q] == ACTIVE);
-            if (!any) break;
+        m += 2 * nsteps;
+        res.actions += 2 * nsteps;
+        res.halfsteps = m;
+        HIP_TRY(hipMemcpyAsync(st.S_host, st.S + size_t(STATUS) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        std::vector<uint8_t> act = st.active;
+        bool any = false, changed = false;
+        for (uint32_t c = 0; c < st.nCols; ++c) {
+            if (!act[c]) continue;
+            bool live = false;
+            for (int j = 0; j < 64 && !live; ++j) live = (st.S_host[64 * c + j] == ACTIVE);
+            if (live) any = true;
+            else { act[c] = 0; changed = true; }
+        }
+        if (!any) break;
+        if (changed) {
+            if (int r = set_active(plan, st, act, err)) return r;
+            scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
         }
     }
     return 0;
 }
 
-// true residual r = b - A x64 with the FP64 action; per-column norms into ws.res; optionally r as rhs32
-template <typename out_t>
-int residual(gfa_plan* plan, SolverState& st, double const* x64, double* y64, out_t* r_out, hipStream_t stream,
-             int& actions, std::string& err) {
-    auto& ws = st.ws;
-    if (int s = capi_action(plan, 64, y64, x64, stream, err)) return s;
+// true residual r = b - A x64 with the FP64 action over the active groups; norms into st.res;
+// optionally r as the next right-hand side
+template <typename T>
+int residual(gfa_plan* plan, SolverState& st, double* y64, T* r_out, int& actions, std::string& err) {
+    DevicePlan& dev = capi_dev(plan);
+    ActiveScope scope(dev, st);
+    if (st.nseg == 0) return 0;
+    hipStream_t const s = st.stream;
+    if (int r = capi_action(plan, 64, y64, st.x64, s, err)) return r;
     actions++;
-    hipLaunchKernelGGL((k_residual<out_t>), dim3(uint32_t(ws.nnzb)), dim3(256), 0, stream, y64, ws.diag, ws.part, r_out);
-    if (int s = reduce(ws, 1, stream, err)) return s;
-    hipLaunchKernelGGL(k_sqrt_copy, dim3((ws.ncol + 255) / 256), dim3(256), 0, stream, ws.res, ws.red, ws.ncol);
+    SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+    hipLaunchKernelGGL((k_residual<T>), dim3(st.nseg), dim3(256), 0, s, y64, st.diag, r_out, g);
+    hipLaunchKernelGGL(k_col_norm, dim3((st.ncol + 255) / 256), dim3(256), 0, s, st.res, st.segpart, st.col_seg,
+                       st.col_active, st.ncol);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -583,8 +812,10 @@ void solver_release(gfa_plan* plan) {
     for (size_t k = 0; k < g_states.size(); ++k) {
         if (g_states[k].first != plan) continue;
         auto* s = g_states[k].second;
-        for (void* p : s->ws.allocations) (void)hipFree(p);
-        if (s->ws.S_host) (void)hipHostFree(s->ws.S_host);
+        drop_graph(*s);
+        for (void* p : s->allocations) (void)hipFree(p);
+        if (s->S_host) (void)hipHostFree(s->S_host);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
         g_states.erase(g_states.begin() + k);
         return;
@@ -598,66 +829,83 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     auto const t0 = std::chrono::steady_clock::now();
     SolverState& st = *state_of(plan, true);
     if (int s = setup(plan, st, err)) return s;
-    auto& ws = st.ws;
-    hipStream_t const stream = (hipStream_t)o.stream;
+    // the caller's stream orders the inputs; the solve itself runs on the solver's own stream
+    HIP_TRY(hipStreamSynchronize((hipStream_t)o.stream));
+    hipStream_t const s = st.stream;
+    bool const eager = o.eager != 0;
     int const check_every = o.check_every > 0 ? o.check_every : 8;
     uint64_t const seed = o.shadow_seed ? o.shadow_seed : 0x5EEDA43Cull;
     gfa_solve_stats S;
     std::memset(&S, 0, sizeof(S));
-    S.total_columns = int32_t(ws.ncol);
-    if (int s = ensure_vectors<double>(ws, 8, 10, err)) return s;
-    double* x64 = (double*)ws.v64[8];   // the FP64 solution
-    double* y64 = (double*)ws.v64[9];   // FP64 action output
-    std::vector<double> resid(ws.ncol);
+    S.total_columns = int32_t(st.ncol);
+    std::vector<double> resid(st.ncol, 0.0);
     auto fetch_resid = [&]() -> int {
-        HIP_TRY(hipMemcpyAsync(resid.data(), ws.res, ws.ncol * sizeof(double), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(hipMemcpyAsync(resid.data(), st.res, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
         return 0;
     };
     auto max_resid = [&]() { double mx = 0; for (double v : resid) mx = std::max(mx, v); return mx; };
+    // groups whose columns all meet the tolerance leave the active set of the next refinement
+    auto unconverged = [&]() {
+        std::vector<uint8_t> a(st.nCols, 0);
+        for (uint32_t c = 0; c < st.nCols; ++c)
+            for (int j = 0; j < 64; ++j) a[c] |= uint8_t(resid[64 * c + j] > o.tolerance);
+        return a;
+    };
     {   // refinement solves stop a column once it is below half the final tolerance
-        std::vector<double> tabs(ws.ncol, o.tolerance * 0.5);
-        HIP_TRY(hipMemcpyAsync(ws.tol_abs, tabs.data(), ws.ncol * sizeof(double), hipMemcpyHostToDevice, stream));
+        std::vector<double> tabs(st.ncol, o.tolerance * 0.5);
+        HIP_TRY(hipMemcpyAsync(st.tol_abs, tabs.data(), st.ncol * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    std::vector<uint8_t> all(st.nCols, 1);
+    if (int r = set_active(plan, st, all, err)) return r;
+    {
+        uint64_t const n = st.nnzb * NBLK;
+        hipLaunchKernelGGL(k_zero, dim3(uint32_t((n / 2 + 255) / 256)), dim3(256), 0, s, st.x64, n);
+        HIP_TRY(hipGetLastError());
     }
 
     if (o.fp_bits == 64) {
-        if (int s = ensure_vectors<double>(ws, 0, 11, err)) return s;
-        double* x = (double*)ws.v64[0];
-        double* rhs64 = (double*)ws.v64[10];
-        TfqmrResult r;
-        if (int s = tfqmr<double>(plan, st, x, nullptr, o.tolerance, nullptr, o.max_iterations, check_every, seed, stream, r, err)) return s;
-        S.iterations += r.halfsteps; S.actions += r.actions;
-        HIP_TRY(hipMemcpyAsync(x64, x, ws.nnzb * NBLK * sizeof(double), hipMemcpyDeviceToDevice, stream));
-        if (int s = residual<double>(plan, st, x64, y64, rhs64, stream, S.actions, err)) return s;
-        if (int s = fetch_resid()) return s;
-        if (o.echo > 0) std::fprintf(stderr, "# gfa_solve fp64: %d half-steps, max residual %.3e\n", r.halfsteps, max_resid());
+        if (int r = ensure_work<double>(st, err)) return r;
+        double* y64 = work<double>(st, 0);  // the work vectors are free between TFQMR runs
+        TfqmrResult tr;
+        if (int r = tfqmr<double>(plan, st, nullptr, o.tolerance, nullptr, o.max_iterations, check_every, seed, eager, tr, err)) return r;
+        S.iterations += tr.halfsteps; S.actions += tr.actions;
+        if (int r = set_active(plan, st, all, err)) return r;
+        if (!st.rhs64 && o.max_refinements > 0) { if (int r = walloc(st, st.rhs64, st.nnzb * NBLK, err)) return r; }
+        if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
+        if (int r = fetch_resid()) return r;
+        if (o.echo > 0) std::fprintf(stderr, "# gfa_solve fp64: %d half-steps, max residual %.3e\n", tr.halfsteps, max_resid());
         // FP64 refinement steps if requested and needed: solve A delta = r, x += delta
         for (int k = 0; k < o.max_refinements && max_resid() > o.tolerance; ++k) {
-            TfqmrResult r2;
-            if (int s = tfqmr<double>(plan, st, x, rhs64, 1e-30, ws.tol_abs, o.max_iterations, check_every, seed + k + 1, stream, r2, err)) return s;
-            S.iterations += r2.halfsteps; S.actions += r2.actions; S.refinements++;
-            hipLaunchKernelGGL((k_accumulate<double>), dim3(uint32_t(ws.nnzb)), dim3(256), 0, stream, x64, x);
-            if (int s = residual<double>(plan, st, x64, y64, rhs64, stream, S.actions, err)) return s;
-            if (int s = fetch_resid()) return s;
+            auto const act = unconverged();
+            if (int r = set_active(plan, st, act, err)) return r;
+            TfqmrResult t2;
+            if (int r = tfqmr<double>(plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every, seed + k + 1, eager, t2, err)) return r;
+            S.iterations += t2.halfsteps; S.actions += t2.actions; S.refinements++;
+            if (int r = set_active(plan, st, act, err)) return r;
+            if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
+            if (int r = fetch_resid()) return r;
         }
     } else {
-        if (int s = ensure_vectors<float>(ws, 0, 9, err)) return s;
-        float* x32 = (float*)ws.v32[0];
-        float* rhs32 = (float*)ws.v32[8];
-        HIP_TRY(hipMemsetAsync(x64, 0, ws.nnzb * NBLK * sizeof(double), stream));
+        if (int r = ensure_work<float>(st, err)) return r;
+        if (!st.rhs32) { if (int r = walloc(st, st.rhs32, st.nnzb * NBLK, err)) return r; }
+        double* y64 = reinterpret_cast<double*>(work<float>(st, 0));  // spans work vectors 0 and 1
         double const floor_rel = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-5;  // gain asked of one FP32 solve
+        std::vector<uint8_t> act = all;
         for (int k = 0; k <= o.max_refinements; ++k) {
-            TfqmrResult r;
-            if (int s = tfqmr<float>(plan, st, x32, k == 0 ? nullptr : rhs32, floor_rel, ws.tol_abs, o.max_iterations,
-                                     check_every, seed + k, stream, r, err)) return s;
-            S.iterations += r.halfsteps; S.actions += r.actions;
+            if (int r = set_active(plan, st, act, err)) return r;
+            TfqmrResult tr;
+            if (int r = tfqmr<float>(plan, st, k == 0 ? nullptr : st.rhs32, floor_rel, st.tol_abs, o.max_iterations,
+                                     check_every, seed + k, eager, tr, err)) return r;
+            S.iterations += tr.halfsteps; S.actions += tr.actions;
             if (k > 0) S.refinements++;
-            hipLaunchKernelGGL((k_accumulate<float>), dim3(uint32_t(ws.nnzb)), dim3(256), 0, stream, x64, x32);
-            if (int s = residual<float>(plan, st, x64, y64, rhs32, stream, S.actions, err)) return s;
-            if (int s = fetch_resid()) return s;
+            if (int r = set_active(plan, st, act, err)) return r;
+            if (int r = residual<float>(plan, st, y64, st.rhs32, S.actions, err)) return r;
+            if (int r = fetch_resid()) return r;
             if (o.echo > 0)
-                std::fprintf(stderr, "# gfa_solve fp32 step %d: %d half-steps, max FP64 residual %.3e\n", k, r.halfsteps, max_resid());
+                std::fprintf(stderr, "# gfa_solve fp32 step %d: %d half-steps, max FP64 residual %.3e\n", k, tr.halfsteps, max_resid());
             if (max_resid() <= o.tolerance) break;
+            act = unconverged();
         }
     }
     S.max_residual = 0;
@@ -666,13 +914,13 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         S.max_residual = std::max(S.max_residual, v);
         S.converged_columns += (v <= o.tolerance);
     }
-    if (o.X_device) HIP_TRY(hipMemcpyAsync(o.X_device, x64, ws.nnzb * NBLK * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    if (o.X_device) HIP_TRY(hipMemcpyAsync(o.X_device, st.x64, st.nnzb * NBLK * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (rho) {
-        double* drho = ws.red;  // 3*ncol doubles >= 2*ncol
-        hipLaunchKernelGGL(k_diag, dim3(ws.ncol / 64), dim3(64), 0, stream, drho, x64, st.subset);
-        HIP_TRY(hipMemcpyAsync(rho, drho, size_t(ws.ncol) * 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        double* out = st.segpart;  // >= 192 doubles per column group, 128 needed
+        hipLaunchKernelGGL(k_diag, dim3(st.nCols), dim3(64), 0, s, out, st.x64, st.subset);
+        HIP_TRY(hipMemcpyAsync(rho, out, size_t(st.ncol) * 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     }
-    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipStreamSynchronize(s));
     S.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     S.flops = double(S.actions) * reference_flops(capi_host(plan), o.fp_bits);
     if (stats) *stats = S;

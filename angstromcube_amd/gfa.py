@@ -52,7 +52,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("fp_bits", C.c_int32), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
                 ("max_refinements", C.c_int32), ("check_every", C.c_int32), ("shadow_seed", C.c_uint64),
                 ("X_device", C.c_void_p), ("stream", C.c_void_p), ("inner_tolerance", C.c_double),
-                ("echo", C.c_int32), ("reserved", C.c_int32)]
+                ("echo", C.c_int32), ("eager", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -83,6 +83,7 @@ def lib():
         L.gfa_set_potential.argtypes = [vp, C.POINTER(C.c_double)]
         L.gfa_set_atom_matrices.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.c_int32]
         L.gfa_set_energy.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+        L.gfa_set_energies.argtypes = [vp, C.POINTER(C.c_double), C.c_double]
         L.gfa_set_kpoint.argtypes = [vp, C.POINTER(C.c_double)]
         L.gfa_action.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_double)]
         L.gfa_action_multiply.argtypes = [vp, C.c_int, vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int,
@@ -247,7 +248,7 @@ class GreenPlan:
         return fl.value
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
-              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0):
+              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False):
         """Solve (H - E S) X = B; returns (rho[nCols, 64] complex, stats dict)."""
         import torch
         o = SolveOpts()
@@ -259,6 +260,7 @@ class GreenPlan:
         o.shadow_seed = shadow_seed
         o.inner_tolerance = inner_tolerance
         o.echo = echo
+        o.eager = 1 if eager else 0
         o.X_device = C.c_void_p(X.data_ptr()) if X is not None else None
         o.stream = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         rho = np.zeros((self.nCols, 64, 2))

@@ -149,3 +149,31 @@ def small_periodic(seed=0xA43C0000):
 def wrap_32(seed=0xA43C0001, natoms=64, ngauss=64):
     """32x32x32 blocks (128^3 grid), periodic (wrapped by a 10 Bohr truncation), 64 atoms numax 5."""
     return make_system(seed, (32, 32, 32), ngauss=ngauss, natoms=natoms, numax=5)
+
+
+def contour_32(seed=0xA43C0003, natoms=128, ngauss=64):
+    """32x32x32 blocks (128^3 grid), periodic (wrapped by truncation), 128 atoms numax 5: the system of the
+    batched energy-contour workload."""
+    return make_system(seed, (32, 32, 32), ngauss=ngauss, natoms=natoms, numax=5)
+
+
+def contour_instances(n_energies=32, n_kpoints=4, radii=(6.0, 8.0, 10.0, 12.0), seed=0xA43C0003):
+    """(E, k, radius) instances of a batched contour integration.
+
+    Energies: Gauss-Legendre nodes x_i of [-1, 1] mapped to angles phi_i = pi (x_i + 1) / 2 on a half ellipse
+    from E = -1 Ha to E_F = 0 Ha, E_i = -1/2 - cos(phi_i)/2 + i (0.005 + 0.295 sin(phi_i)), so Im E runs from
+    0.005 Ha (slow solves) to 0.3 Ha (fast ones); the weights are the Gauss-Legendre weights times dE/dphi.
+    k-points: n_kpoints seeded points in the first Brillouin zone (units of 2 pi / cell).  Radius: drawn per
+    instance uniformly from `radii`.  Returns a list of dicts with keys E, weight, k, radius."""
+    x, w = np.polynomial.legendre.leggauss(n_energies)
+    phi = 0.5 * np.pi * (x + 1.0)
+    E = -0.5 - 0.5 * np.cos(phi) + 1j * (0.005 + 0.295 * np.sin(phi))
+    dE = (0.5 * np.sin(phi) + 1j * 0.295 * np.cos(phi)) * 0.5 * np.pi
+    rng = SplitMix64(seed ^ 0x6B70)
+    kpts = [tuple(rng.uniform(-0.5, 0.5) for _ in range(3)) for _ in range(n_kpoints)]
+    out = []
+    for k in kpts:
+        for i in range(n_energies):
+            r = radii[min(len(radii) - 1, int(rng.uniform() * len(radii)))]
+            out.append(dict(E=complex(E[i]), weight=complex(w[i] * dE[i]), k=k, radius=float(r)))
+    return out

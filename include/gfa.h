@@ -151,7 +151,7 @@ typedef struct {
     void* stream;               /* hipStream_t or NULL                                                */
     double inner_tolerance;     /* fp_bits 32: residual reduction asked of one FP32 solve (0: 1e-5)   */
     int32_t echo;               /* >0: one line per inner solve on stderr                             */
-    int32_t reserved;
+    int32_t eager;              /* 0: replay each iteration from a captured HIP graph; 1: plain launches */
 } gfa_solve_opts;
 
 typedef struct {

@@ -400,13 +400,17 @@ void oracle_action(oracle_state* s, int which, double* y, const double* x) {
     if (which < 0 || which == 4) dyadic(s, y, x);
 }
 
-/* shadow residual: SplitMix64 finaliser of (seed, index), mapped to [-1, 1) */
+/* shadow residual r0*: murmur3's 32-bit finaliser (Appleby, MurmurHash3) of a mix of seed and element
+ * index; the top 24 bits as a signed fraction in [-1, 1), exact in FP32 and FP64 */
 double oracle_shadow(uint64_t seed, uint64_t idx) {
-    uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    uint32_t h = (uint32_t)idx + 0x9E3779B9u * ((uint32_t)(idx >> 32) + 1u) + (uint32_t)seed * 0x27D4EB2Fu +
+                 (uint32_t)(seed >> 32);
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return (double)((int32_t)h >> 8) * (1.0 / 8388608.0);
 }
 
 /* ---- TFQMR, Saad Algorithm 7.8, per column; the same scalar formulas as the device solver ---- */

@@ -43,6 +43,9 @@ def test_batched_plan_columns_equal_single_plans():
     pb = batched(sysm)
     assert pb.nCols == 3
     assert tuple(pb.sizes.bc) == (5, 5, 5)
+    pb.set_inputs(np.array(ENERGIES), K)                 # host-side: one energy per column group
+    with pytest.raises(G.GfaError, match="dVol"):
+        pb.update_energies(np.array(ENERGIES), dVol=0.0)
     sub = pb.export("SUBSET")
     assert len(set(sub.tolist())) == 3          # a repeated source still gets its own diagonal block
     cols, rows, pos = pb.export("COLINDX"), pb.export("ROWINDX"), pb.export("ROW_POS")
