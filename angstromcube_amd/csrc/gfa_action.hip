@@ -17,12 +17,7 @@
 //                     written once.  The block is staged in LDS; neighbour blocks come through L2.
 #include "gfa_device.hpp"
 
-#ifndef LOCAL_LB
-#define LOCAL_LB 2   // kinetic lines per thread loaded together
-#endif
-#ifndef LOCAL_JW
-#define LOCAL_JW 32  // columns per work-group of k_local
-#endif
+
 
 #include <algorithm>
 #include <cmath>
@@ -162,10 +157,13 @@ __global__ __launch_bounds__(128) void k_sho_project(
 
 // ---------------------------------------------------------------------------------------------
 // 2. per (atom, column): collect images with phase, multiply (h - E s) dVol, broadcast with
-//    conjugate phase.  64 threads = j; both planes per thread; coefficients staged in LDS.
+//    conjugate phase.  ATOM_WAVES waves; lane = column j, both planes per thread.  The waves share
+//    the collected coefficients (LDS up to 64 coefficients, numax <= 5, else a global scratch row):
+//    coefficient s is collected by wave s % ATOM_WAVES, output row i computed by wave i % ATOM_WAVES.
 // ---------------------------------------------------------------------------------------------
+#define ATOM_WAVES 8
 template <typename real_t>
-__global__ __launch_bounds__(64) void k_sho_atoms(
+__global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
     real_t* __restrict__ dcoef, real_t const* __restrict__ partial, uint32_t const* __restrict__ chunk_coef,
     uint32_t const* __restrict__ pair_chunk, uint32_t const* __restrict__ sum_start,
     uint32_t const* __restrict__ sum_index, uint32_t const* __restrict__ image_starts,
@@ -176,34 +174,36 @@ __global__ __launch_bounds__(64) void k_sho_atoms(
     extern __shared__ double lds[];
     uint32_t const a = blockIdx.x, c = blockIdx.y;
     if (col_active && !col_active[c]) return;  // a retired column group
-    int const j = threadIdx.x;
+    int const j = threadIdx.x & 63, w = threadIdx.x >> 6;
     int const L = atom_lmax[a];
     int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
-    // atom coefficients of this column, private to each thread: in LDS up to 64 coefficients
-    // (numax <= 5), else in a global scratch row
     double* cre = (nc <= 64) ? lds : cscratch + (size_t(atom_starts[a]) * nCols + size_t(c) * nc) * 128;
     double* cim = cre + nc * 64;
-    for (int s = 0; s < nc; ++s) { cre[s * 64 + j] = 0; cim[s * 64 + j] = 0; }
-    for (uint32_t k = sum_start[a]; k < sum_start[a + 1]; ++k) {
-        uint32_t const img = sum_index[k];
-        double const ph_re = image_phase[2 * img], ph_im = image_phase[2 * img + 1];
-        uint32_t const c0 = pair_chunk[size_t(img) * nCols + c], c1 = pair_chunk[size_t(img) * nCols + c + 1];
-        for (int s = 0; s < nc; ++s) {
+    uint32_t const k0 = sum_start[a], k1 = sum_start[a + 1];
+    for (int s = w; s < nc; s += ATOM_WAVES) {
+        double ar = 0, ai = 0;
+        for (uint32_t k = k0; k < k1; ++k) {
+            uint32_t const img = sum_index[k];
+            double const ph_re = image_phase[2 * img], ph_im = image_phase[2 * img + 1];
+            uint32_t const c0 = pair_chunk[size_t(img) * nCols + c], c1 = pair_chunk[size_t(img) * nCols + c + 1];
             double vr = 0, vi = 0;
             for (uint32_t ch = c0; ch < c1; ++ch) {
                 real_t const* src = partial + (size_t(chunk_coef[ch]) + s) * 128 + j;
                 vr += double(src[0]);
                 vi += double(src[64]);
             }
-            cre[s * 64 + j] += ph_re * vr - ph_im * vi;
-            cim[s * 64 + j] += ph_re * vi + ph_im * vr;
+            ar += ph_re * vr - ph_im * vi;
+            ai += ph_re * vi + ph_im * vr;
         }
+        cre[s * 64 + j] = ar;
+        cim[s * 64 + j] = ai;
     }
+    __syncthreads();
     double const* HS = atom_mat + atom_mat_off[a];  // [2][nc][nc]: h dVol, s dVol
     double const Er = E_col[2 * c], Ei = E_col[2 * c + 1];
     // d = M c with M = (h - E s) dVol of this column's energy, complex double accumulation in ascending
     // column order (include/green_dyadic.hxx:800-819, src/green_function.cxx:97-98)
-    for (int i = 0; i < nc; ++i) {
+    for (int i = w; i < nc; i += ATOM_WAVES) {
         double dr = 0, di = 0;
         for (int k = 0; k < nc; ++k) {
             double const hd = HS[i * nc + k], sd = HS[nc * nc + i * nc + k];
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(64) void k_sho_atoms(
             dr += mr * cr - mi * ci;
             di += mr * ci + mi * cr;
         }
-        for (uint32_t k = sum_start[a]; k < sum_start[a + 1]; ++k) {
+        for (uint32_t k = k0; k < k1; ++k) {
             uint32_t const img = sum_index[k];
             double const ph_re = image_phase[2 * img], ph_im = -image_phase[2 * img + 1];
             real_t* dst = dcoef + ((size_t(image_starts[img]) + i) * nCols + c) * 128 + j;
@@ -317,12 +317,11 @@ __device__ __forceinline__ void sho_add_atom(real_t* __restrict__ ys, real_t con
 // (xs) and the output accumulator (ys); passes: potential, kinetic x, y, z, SHO add-back, store.
 // In the kinetic passes a thread owns 4 lines (4 points each) of one (column, plane); neighbour
 // values come straight from global memory with branch-free addressing (absent -> zero block).
-template <typename real_t, int NH, bool IMG, int JW>
+template <typename real_t, int NH, bool IMG, int JW, int LB>
 __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
     constexpr int T = 8 * JW;
     constexpr int NW = 4 * (2 * NH + 1);   // points in a line window
     constexpr int NSLICE = 64 / JW;
-    constexpr int LB = LOCAL_LB;
     __shared__ real_t xs[2 * 64 * JW];
     __shared__ real_t ys[2 * 64 * JW];
     __shared__ real_t H[8][3][4];
@@ -718,6 +717,18 @@ int device_upload_values(Plan const& p, DevicePlan& d, std::string& err) {
     return 0;
 }
 
+// k_local tile shapes: JW columns per work-group (16 or 32), LB kinetic lines per thread in flight (1, 2, 4)
+template <typename real_t, int NH, bool IMG>
+static void launch_local(int jw, int lb, dim3 g, hipStream_t stream, LocalArgs<real_t> const& a) {
+#define KL(JW_, LB_) hipLaunchKernelGGL((k_local<real_t, NH, IMG, JW_, LB_>), g, dim3(8 * JW_), 0, stream, a)
+    if (jw == 16) {
+        if (lb == 1) KL(16, 1); else if (lb == 4) KL(16, 4); else KL(16, 2);
+    } else {
+        if (lb == 1) KL(32, 1); else if (lb == 4) KL(32, 4); else KL(32, 2);
+    }
+#undef KL
+}
+
 template <typename real_t>
 static Workspace<real_t>& workspace(DevicePlan& d);
 template <> Workspace<double>& workspace<double>(DevicePlan& d) { return d.ws64; }
@@ -725,13 +736,13 @@ template <> Workspace<float>& workspace<float>(DevicePlan& d) { return d.ws32; }
 
 template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
-                  std::string& err) {
+                  std::string& err, int parts) {
     auto& ws = workspace<real_t>(d);
     if (d.nAtomImages > 0 && !ws.dcoef) {
         if (int r = dalloc(d, ws.partial, size_t(d.total_chunk_coefs) * 128, err)) return r;
         if (int r = dalloc(d, ws.dcoef, size_t(d.natomcoeffs) * d.nCols * 128, err)) return r;
     }
-    if (d.nAtomImages > 0 && d.nchunks > 0) {
+    if ((parts & ACTION_PROJECT) && d.nAtomImages > 0 && d.nchunks > 0) {
         for (int L = 0; L <= 7; ++L) {
             uint32_t const n = d.bucket_start[L + 1] - d.bucket_start[L];
             if (!n) continue;
@@ -752,12 +763,13 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
             if (int r = dalloc(d, d.cscratch, size_t(d.ncoeffs) * d.nCols * 128, err)) return r;
         }
         size_t const lds = size_t(2) * std::min<uint32_t>(d.max_atom_nc, 64) * 64 * sizeof(double);
-        hipLaunchKernelGGL((k_sho_atoms<real_t>), dim3(d.nAtoms, d.nCols), dim3(64), lds, stream, ws.dcoef,
+        hipLaunchKernelGGL((k_sho_atoms<real_t>), dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), lds, stream, ws.dcoef,
                            ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.image_starts,
                            d.atom_lmax, d.atom_hs, d.atom_mat_off, d.image_phase, d.nCols, d.cscratch,
                            d.atom_starts, d.E_col, d.col_active);
         HIP_TRY(hipGetLastError());
     }
+    if (!(parts & ACTION_LOCAL)) return 0;
     LocalArgs<real_t> a;
     a.y = y; a.x = x; a.dcoef = ws.dcoef; a.veff = d.veff; a.nbr = d.nbr; a.V = d.V;
     a.add_start = d.add_start; a.add_index = d.add_index; a.colindx = d.colindx; a.rowindx = d.rowindx;
@@ -782,21 +794,22 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     }
     for (int k = 0; k < 4; ++k) a.hgrid[k] = s.hgrid[k];
     // work-groups are dealt round-robin to the 8 XCDs: give each XCD a contiguous range of
-    // (block, column half) items so that neighbour blocks meet in the same L2
-    constexpr int JW = LOCAL_JW;
+    // (block, column slice) items so that neighbour blocks meet in the same L2
+    int const v = sizeof(real_t) == 8 ? 0 : 1;
+    int const JW = d.tune_jw[v], LB = d.tune_lb[v];
     if (a.nitems == 0) return 0;
-    uint64_t const items = (64 / JW) * a.nitems;
+    uint64_t const items = uint64_t(64 / JW) * a.nitems;
     uint64_t const per = (items + 7) / 8;
     a.per_xcd = uint32_t(per);
     uint64_t const grid = per * 8;
     if (grid > 0x7fffffffull) { err = "too many blocks for one launch"; return 1; }
-    dim3 const g{uint32_t(grid), 1, 1}, b{8 * JW, 1, 1};
+    dim3 const g{uint32_t(grid), 1, 1};
     if (d.nh_max == 2) {
-        if (d.has_images) hipLaunchKernelGGL((k_local<real_t, 2, true, JW>), g, b, 0, stream, a);
-        else hipLaunchKernelGGL((k_local<real_t, 2, false, JW>), g, b, 0, stream, a);
+        if (d.has_images) launch_local<real_t, 2, true>(JW, LB, g, stream, a);
+        else launch_local<real_t, 2, false>(JW, LB, g, stream, a);
     } else {
-        if (d.has_images) hipLaunchKernelGGL((k_local<real_t, 1, true, JW>), g, b, 0, stream, a);
-        else hipLaunchKernelGGL((k_local<real_t, 1, false, JW>), g, b, 0, stream, a);
+        if (d.has_images) launch_local<real_t, 1, true>(JW, LB, g, stream, a);
+        else launch_local<real_t, 1, false>(JW, LB, g, stream, a);
     }
     HIP_TRY(hipGetLastError());
     if (d.n_wide_pairs > 0) {
@@ -807,7 +820,43 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     return 0;
 }
 
-template int device_action<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, std::string&);
-template int device_action<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, std::string&);
+template int device_action<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, std::string&, int);
+template int device_action<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, std::string&, int);
+
+// Times every k_local tile shape on the caller's vectors (the whole action, reps launches each, HIP events on
+// `stream`) and keeps the fastest for this precision.  y is overwritten.
+template <typename real_t>
+int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream, int reps,
+                    double* best_ms, std::string& err) {
+    int const v = sizeof(real_t) == 8 ? 0 : 1;
+    static int const JWS[2] = {16, 32}, LBS[3] = {1, 2, 4};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e300;
+    int bjw = d.tune_jw[v], blb = d.tune_lb[v];
+    for (int jw : JWS)
+        for (int lb : LBS) {
+            d.tune_jw[v] = jw; d.tune_lb[v] = lb;
+            if (int r = device_action<real_t>(d, s, y, x, stream, err)) return r;  // warm-up
+            HIP_TRY(hipEventRecord(e0, stream));
+            for (int k = 0; k < reps; ++k)
+                if (int r = device_action<real_t>(d, s, y, x, stream, err)) return r;
+            HIP_TRY(hipEventRecord(e1, stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            double const t = double(ms) / std::max(1, reps);
+            if (t < best) { best = t; bjw = jw; blb = lb; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    d.tune_jw[v] = bjw; d.tune_lb[v] = blb;
+    d.tuned[v] = true;
+    if (best_ms) *best_ms = best;
+    return 0;
+}
+template int device_autotune<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, int, double*, std::string&);
+template int device_autotune<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, int, double*, std::string&);
 
 }  // namespace gfa

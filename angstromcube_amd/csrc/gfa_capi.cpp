@@ -71,10 +71,16 @@ gfa::ActionScalars scalars(gfa::Plan const& h) {
 
 namespace gfa {
 // used by the solver: run one action on a plan handle
-int capi_action(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, std::string& err) {
+int capi_action(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, std::string& err, int parts) {
     auto const s = scalars(p->host);
-    if (fp_bits == 64) return device_action<double>(p->dev, s, (double*)y, (double const*)x, stream, err);
-    return device_action<float>(p->dev, s, (float*)y, (float const*)x, stream, err);
+    if (fp_bits == 64) return device_action<double>(p->dev, s, (double*)y, (double const*)x, stream, err, parts);
+    return device_action<float>(p->dev, s, (float*)y, (float const*)x, stream, err, parts);
+}
+int capi_autotune(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, int reps, double* best_ms,
+                  std::string& err) {
+    auto const s = scalars(p->host);
+    if (fp_bits == 64) return device_autotune<double>(p->dev, s, (double*)y, (double const*)x, stream, reps, best_ms, err);
+    return device_autotune<float>(p->dev, s, (float*)y, (float const*)x, stream, reps, best_ms, err);
 }
 Plan const& capi_host(gfa_plan const* p) { return p->host; }
 DevicePlan& capi_dev(gfa_plan* p) { return p->dev; }
@@ -277,6 +283,50 @@ int gfa_action(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream
         if (int s = gfa::capi_action(plan, fp_bits, y, x, (hipStream_t)stream, err))
             return fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
         if (nflops) *nflops = gfa::reference_flops(plan->host, fp_bits);
+        return GFA_OK;
+    });
+}
+
+int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* x, void* stream) {
+    return guarded([&]() -> int {
+        if (!plan || !y || !x) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (part < 1 || part > 3) return fail(GFA_ERR_ARGUMENT, "part must be 1 (projection), 2 (local) or 3 (both)");
+        if (y == x) return fail(GFA_ERR_ARGUMENT, "the action cannot run in place");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energy before applying the action");
+        if (int s = ensure_device(plan)) return s;
+        std::string err;
+        if (int s = gfa::capi_action(plan, fp_bits, y, x, (hipStream_t)stream, err, part))
+            return fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
+        return GFA_OK;
+    });
+}
+
+int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int columns, int lines) {
+    return guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if ((columns != 16 && columns != 32) || (lines != 1 && lines != 2 && lines != 4))
+            return fail(GFA_ERR_ARGUMENT, "tile shapes: 16 or 32 columns, 1, 2 or 4 lines");
+        int const v = fp_bits == 64 ? 0 : 1;
+        plan->dev.tune_jw[v] = columns;
+        plan->dev.tune_lb[v] = lines;
+        plan->dev.tuned[v] = true;
+        gfa::solver_release(plan);  // a captured solver graph holds the previous shape
+        return GFA_OK;
+    });
+}
+
+int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, int reps, double* best_ms) {
+    return guarded([&]() -> int {
+        if (!plan || !y || !x) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (y == x) return fail(GFA_ERR_ARGUMENT, "the action cannot run in place");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energy before tuning the action");
+        if (int s = ensure_device(plan)) return s;
+        std::string err;
+        if (gfa::capi_autotune(plan, fp_bits, y, x, (hipStream_t)stream, reps > 0 ? reps : 3, best_ms, err))
+            return fail(GFA_ERR_DEVICE, err);
         return GFA_OK;
     });
 }

@@ -35,6 +35,9 @@ struct DevicePlan {
     int nh_max = 1;                    // neighbour blocks per side in the widest stencil
     bool has_images = false;           // some finite-difference halo is a periodic image
     int fd_range[3] = {4, 4, 4};
+    // k_local tile shape per precision (index 0: FP64, 1: FP32), set by the autotuner
+    int tune_jw[2] = {32, 32}, tune_lb[2] = {2, 2};
+    bool tuned[2] = {false, false};
 
     // index data
     uint16_t* colindx = nullptr;       // [nnzb]
@@ -97,11 +100,17 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err);
 int device_upload_values(Plan const& p, DevicePlan& d, std::string& err);
 void device_free(DevicePlan& d);
 
+// parts of the action: the SHO projection and atom-matrix multiply (coefficients into the workspace), and
+// the local kernel (potential, kinetic, add-back of the workspace coefficients) that writes y
+enum { ACTION_PROJECT = 1, ACTION_LOCAL = 2, ACTION_ALL = 3 };
 template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
-                  std::string& err);
+                  std::string& err, int parts = ACTION_ALL);
 
 // flop count of one action as the reference counts it (include/green_action.hxx:154-175)
+template <typename real_t>
+int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream, int reps,
+                    double* best_ms, std::string& err);
 double reference_flops(Plan const& p, int fp_bits);
 
 }  // namespace gfa

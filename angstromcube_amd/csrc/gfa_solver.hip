@@ -393,12 +393,34 @@ __global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, 
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ double& F(double* S, uint32_t ncol, int f, uint32_t q) { return S[uint64_t(f) * ncol + q]; }
 
-__device__ __forceinline__ double fold(double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
-                                       uint32_t q, int c) {
-    uint32_t const col = q >> 6, j = q & 63;
-    double s = 0;
-    for (uint32_t k = col_seg[col]; k < col_seg[col + 1]; ++k) s += segpart[(uint64_t(k) * 3 + c) * 64 + j];
-    return s;
+constexpr int FOLD_P = 8;  // waves folding the segments of one column group
+
+// sums of the partial fields f[0..NC) of column group c over its segments: wave p adds segments
+// p, p + FOLD_P, ...; the waves' sums are then added in wave order (deterministic).  Valid in threads
+// tid < 64 (column j = tid).
+template <int NC>
+__device__ __forceinline__ void fold_group(double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
+                                           uint32_t c, int const (&f)[NC], double (&tot)[NC]) {
+    __shared__ double sh[NC][FOLD_P][64];
+    int const tid = threadIdx.x, j = tid & 63, part = tid >> 6;
+    uint32_t const k0 = col_seg[c], k1 = col_seg[c + 1];
+    double acc[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) acc[n] = 0;
+    for (uint32_t k = k0 + part; k < k1; k += FOLD_P)
+#pragma unroll
+        for (int n = 0; n < NC; ++n) acc[n] += segpart[(uint64_t(k) * 3 + f[n]) * 64 + j];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) sh[n][part][j] = acc[n];
+    __syncthreads();
+    if (tid < 64) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            double sum = 0;
+            for (int p = 0; p < FOLD_P; ++p) sum += sh[n][p][j];
+            tot[n] = sum;
+        }
+    }
 }
 
 __device__ __forceinline__ void freeze_step(double* S, uint32_t ncol, uint32_t q) {
@@ -406,24 +428,33 @@ __device__ __forceinline__ void freeze_step(double* S, uint32_t ncol, uint32_t q
     for (int f = ETAE_RE; f <= BETA_IM; ++f) F(S, ncol, f, q) = 0;
 }
 
+// The scalar kernels run one work-group of 64 * FOLD_P threads per column group; threads tid < 64
+// update column q = 64 c + tid.
+
 // after k_init: |b|^2, (r0*, b); columns of inactive groups start as converged
-__global__ void k_scalar_init(double* __restrict__ S, double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
-                              uint8_t const* __restrict__ col_active, uint32_t ncol, double tol_rel,
-                              double const* __restrict__ tol_abs) {
-    uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ncol) return;
-    if (!col_active[q >> 6]) {
-        F(S, ncol, STATUS, q) = CONVERGED;
-        F(S, ncol, MSTEP, q) = 0;
-        freeze_step(S, ncol, q);
+__global__ __launch_bounds__(64 * FOLD_P) void k_scalar_init(double* __restrict__ S, double const* __restrict__ segpart,
+                                                          uint32_t const* __restrict__ col_seg,
+                                                          uint8_t const* __restrict__ col_active, uint32_t ncol,
+                                                          double tol_rel, double const* __restrict__ tol_abs) {
+    uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
+    if (!col_active[c]) {
+        if (threadIdx.x < 64) {
+            F(S, ncol, STATUS, q) = CONVERGED;
+            F(S, ncol, MSTEP, q) = 0;
+            freeze_step(S, ncol, q);
+        }
         return;
     }
-    double const bn = sqrt(fold(segpart, col_seg, q, 0));
+    int const f[3] = {0, 1, 2};
+    double t[3];
+    fold_group<3>(segpart, col_seg, c, f, t);
+    if (threadIdx.x >= 64) return;
+    double const bn = sqrt(t[0]);
     F(S, ncol, BNORM, q) = bn;
     F(S, ncol, TAU, q) = bn;
     F(S, ncol, THETA, q) = 0;
-    F(S, ncol, RHO_RE, q) = fold(segpart, col_seg, q, 1);
-    F(S, ncol, RHO_IM, q) = fold(segpart, col_seg, q, 2);
+    F(S, ncol, RHO_RE, q) = t[1];
+    F(S, ncol, RHO_IM, q) = t[2];
     double const tol = fmax(tol_rel * bn, tol_abs ? tol_abs[q] : 0.0);
     F(S, ncol, TOL, q) = tol;
     F(S, ncol, MSTEP, q) = 0;
@@ -433,12 +464,17 @@ __global__ void k_scalar_init(double* __restrict__ S, double const* __restrict__
 }
 
 // start of a step: alpha = rho / (r0*, v); a column that is no longer active gets zero coefficients
-__global__ void k_scalar_alpha(double* __restrict__ S, double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
-                               uint32_t ncol) {
-    uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ncol) return;
+__global__ __launch_bounds__(64 * FOLD_P) void k_scalar_alpha(double* __restrict__ S, double const* __restrict__ segpart,
+                                                           uint32_t const* __restrict__ col_seg,
+                                                           uint8_t const* __restrict__ col_active, uint32_t ncol) {
+    uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
+    if (!col_active[c]) return;
+    int const f[2] = {0, 1};
+    double t[2];
+    fold_group<2>(segpart, col_seg, c, f, t);
+    if (threadIdx.x >= 64) return;
     if (F(S, ncol, STATUS, q) != ACTIVE) { freeze_step(S, ncol, q); return; }
-    double const sr = fold(segpart, col_seg, q, 0), si = fold(segpart, col_seg, q, 1);
+    double const sr = t[0], si = t[1];
     double const den = sr * sr + si * si;
     if (!(den > 0)) { F(S, ncol, STATUS, q) = BREAKDOWN; freeze_step(S, ncol, q); return; }
     double const rr = F(S, ncol, RHO_RE, q), ri = F(S, ncol, RHO_IM, q);
@@ -449,10 +485,18 @@ __global__ void k_scalar_alpha(double* __restrict__ S, double const* __restrict_
 // after a w update: theta, c, tau, eta, the d coefficient, convergence; on odd half-steps also rho, beta.
 // Even half-steps write the E fields, odd ones the O fields (both are used by k_dx2).
 template <int ODD>
-__global__ void k_scalar_theta(double* __restrict__ S, double const* __restrict__ segpart,
-                               uint32_t const* __restrict__ col_seg, uint32_t ncol) {
-    uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ncol) return;
+__global__ __launch_bounds__(64 * FOLD_P) void k_scalar_theta(double* __restrict__ S, double const* __restrict__ segpart,
+                                                           uint32_t const* __restrict__ col_seg,
+                                                           uint8_t const* __restrict__ col_active, uint32_t ncol) {
+    uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
+    if (!col_active[c]) return;
+    constexpr int NC = ODD ? 3 : 1;
+    int f[NC];
+    double t[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) f[n] = n;
+    fold_group<NC>(segpart, col_seg, c, f, t);
+    if (threadIdx.x >= 64) return;
     int const FE = ODD ? ETAO_RE : ETAE_RE, FC = ODD ? CDO_RE : CDE_RE, FP = ODD ? ETAE_RE : ETAO_RE;
     if (F(S, ncol, STATUS, q) != ACTIVE) {
         F(S, ncol, FE, q) = 0; F(S, ncol, FE + 1, q) = 0;
@@ -472,18 +516,18 @@ __global__ void k_scalar_theta(double* __restrict__ S, double const* __restrict_
         ci = (ti * ar - tr * ai) / a2;
     }
     double const tau = F(S, ncol, TAU, q);
-    double const thn = sqrt(fold(segpart, col_seg, q, 0)) / tau;
-    double const c = 1.0 / sqrt(1.0 + thn * thn);
-    double const taun = tau * thn * c;
+    double const thn = sqrt(t[0]) / tau;
+    double const cc = 1.0 / sqrt(1.0 + thn * thn);
+    double const taun = tau * thn * cc;
     F(S, ncol, FC, q) = cr; F(S, ncol, FC + 1, q) = ci;
     F(S, ncol, THETA, q) = thn;
     F(S, ncol, TAU, q) = taun;
-    F(S, ncol, FE, q) = c * c * ar;
-    F(S, ncol, FE + 1, q) = c * c * ai;
+    F(S, ncol, FE, q) = cc * cc * ar;
+    F(S, ncol, FE + 1, q) = cc * cc * ai;
     F(S, ncol, MSTEP, q) = m + 1;
     if (taun * sqrt(m + 2) <= F(S, ncol, TOL, q)) F(S, ncol, STATUS, q) = CONVERGED;  // x still gets this step
     if (ODD) {
-        double const rnr = fold(segpart, col_seg, q, 1), rni = fold(segpart, col_seg, q, 2);
+        double const rnr = t[ODD ? 1 : 0], rni = t[ODD ? 2 : 0];
         double const rr = F(S, ncol, RHO_RE, q), ri = F(S, ncol, RHO_IM, q);
         double const den = rr * rr + ri * ri;
         if (!(den > 0) || !(rnr * rnr + rni * rni > 0)) {
@@ -503,11 +547,15 @@ __global__ void k_scalar_theta(double* __restrict__ S, double const* __restrict_
 }
 
 // residual norms of the active column groups (others keep their value)
-__global__ void k_col_norm(double* __restrict__ res, double const* __restrict__ segpart, uint32_t const* __restrict__ col_seg,
-                           uint8_t const* __restrict__ col_active, uint32_t ncol) {
-    uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ncol || !col_active[q >> 6]) return;
-    res[q] = sqrt(fold(segpart, col_seg, q, 0));
+__global__ __launch_bounds__(64 * FOLD_P) void k_col_norm(double* __restrict__ res, double const* __restrict__ segpart,
+                                                       uint32_t const* __restrict__ col_seg,
+                                                       uint8_t const* __restrict__ col_active) {
+    uint32_t const c = blockIdx.x;
+    if (!col_active[c]) return;
+    int const f[1] = {0};
+    double t[1];
+    fold_group<1>(segpart, col_seg, c, f, t);
+    if (threadIdx.x < 64) res[64 * c + threadIdx.x] = sqrt(t[0]);
 }
 
 // x = 0 over whole blocks (a fill kernel rather than a memset, so that it may sit in a captured graph)
@@ -560,6 +608,7 @@ struct SolverState {
     double* S_host = nullptr;               // pinned copy of the status field
     double* res = nullptr;                  // [ncol] true residual norms
     double* tol_abs = nullptr;              // [ncol]
+    std::vector<double> col_halfsteps;      // [ncol] half-steps of every column in this solve
     // vectors
     double* x64 = nullptr;                  // the FP64 solution
     double* rhs64 = nullptr;                // FP64 residual (refinement right-hand side)
@@ -703,13 +752,13 @@ template <typename T>
 int enqueue_step(gfa_plan* plan, SolverState& st, Vecs<T> const& V, uint64_t seed, hipStream_t s, std::string& err) {
     int const fp = sizeof(T) == 8 ? 64 : 32;
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
-    dim3 const gb(st.nseg), tb(256), gs((st.ncol + 255) / 256), ts(256);
-    hipLaunchKernelGGL(k_scalar_alpha, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.ncol);
+    dim3 const gb(st.nseg), tb(256), gs(st.nCols), ts(64 * FOLD_P);
+    hipLaunchKernelGGL(k_scalar_alpha, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
     hipLaunchKernelGGL((k_even<T>), gb, tb, 0, s, V.un, V.u, V.v, V.w, V.Aeven, st.S, st.ncol, g);
-    hipLaunchKernelGGL((k_scalar_theta<0>), gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.ncol);
+    hipLaunchKernelGGL((k_scalar_theta<0>), gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
     if (int r = capi_action(plan, fp, V.Aodd, V.un, s, err)) return r;
     hipLaunchKernelGGL((k_odd<T>), gb, tb, 0, s, V.w, V.Aodd, st.S, st.ncol, g, seed);
-    hipLaunchKernelGGL((k_scalar_theta<1>), gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.ncol);
+    hipLaunchKernelGGL((k_scalar_theta<1>), gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
     hipLaunchKernelGGL((k_dx2<T>), gb, tb, 0, s, V.d, st.x64, V.u, V.un, V.w, st.S, st.ncol, g);
     if (int r = capi_action(plan, fp, V.Aeven, V.u, s, err)) return r;
     hipLaunchKernelGGL((k_vnext<T>), gb, tb, 0, s, V.v, V.Aeven, V.Aodd, st.S, st.ncol, g, seed);
@@ -721,13 +770,13 @@ int enqueue_step(gfa_plan* plan, SolverState& st, Vecs<T> const& V, uint64_t see
 // Column groups whose columns all stopped are retired from the active set as the solve goes.
 template <typename T>
 int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
-          int check_every, uint64_t seed, bool eager, TfqmrResult& res, std::string& err) {
+          int check_every, uint64_t seed, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
     if (int s = ensure_work<T>(st, err)) return s;
     Vecs<T> V{work<T>(st, 0), work<T>(st, 1), work<T>(st, 2), work<T>(st, 3), work<T>(st, 4), work<T>(st, 5), work<T>(st, 6)};
     int const fp = sizeof(T) == 8 ? 64 : 32;
     hipStream_t const s = st.stream;
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
-    dim3 const gs((st.ncol + 255) / 256), ts(256);
+    dim3 const gs(st.nCols), ts(64 * FOLD_P);
     DevicePlan& dev = capi_dev(plan);
     ActiveScope scope(dev, st);
     if (st.nseg == 0) return 0;
@@ -740,6 +789,10 @@ int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double 
     HIP_TRY(hipMemcpyAsync(V.Aeven, V.v, st.nnzb * NBLK * sizeof(T), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL((k_dot_shadow<T>), dim3(st.nseg), dim3(256), 0, s, V.v, g, seed);
     HIP_TRY(hipGetLastError());
+    if (autotune && !dev.tuned[sizeof(T) == 8 ? 0 : 1]) {  // tile shapes, before any graph is captured
+        drop_graph(st);
+        if (int r = capi_autotune(plan, fp, V.Aodd, V.u, s, 2, nullptr, err)) return r;
+    }
 
     int const steps_per_check = std::max(1, check_every / 2);
     int m = 0;
@@ -785,6 +838,12 @@ int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double 
             scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
         }
     }
+    {   // per-column half-step counts of this run (columns of inactive groups count 0)
+        std::vector<double> ms(st.ncol);
+        HIP_TRY(hipMemcpyAsync(ms.data(), st.S + size_t(MSTEP) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (uint32_t q = 0; q < st.ncol; ++q) st.col_halfsteps[q] += ms[q];
+    }
     return 0;
 }
 
@@ -800,8 +859,8 @@ int residual(gfa_plan* plan, SolverState& st, double* y64, T* r_out, int& action
     actions++;
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
     hipLaunchKernelGGL((k_residual<T>), dim3(st.nseg), dim3(256), 0, s, y64, st.diag, r_out, g);
-    hipLaunchKernelGGL(k_col_norm, dim3((st.ncol + 255) / 256), dim3(256), 0, s, st.res, st.segpart, st.col_seg,
-                       st.col_active, st.ncol);
+    hipLaunchKernelGGL(k_col_norm, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.res, st.segpart, st.col_seg,
+                       st.col_active);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -857,6 +916,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         HIP_TRY(hipMemcpyAsync(st.tol_abs, tabs.data(), st.ncol * sizeof(double), hipMemcpyHostToDevice, s));
     }
     std::vector<uint8_t> all(st.nCols, 1);
+    st.col_halfsteps.assign(st.ncol, 0.0);
     if (int r = set_active(plan, st, all, err)) return r;
     {
         uint64_t const n = st.nnzb * NBLK;
@@ -868,7 +928,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         if (int r = ensure_work<double>(st, err)) return r;
         double* y64 = work<double>(st, 0);  // the work vectors are free between TFQMR runs
         TfqmrResult tr;
-        if (int r = tfqmr<double>(plan, st, nullptr, o.tolerance, nullptr, o.max_iterations, check_every, seed, eager, tr, err)) return r;
+        if (int r = tfqmr<double>(plan, st, nullptr, o.tolerance, nullptr, o.max_iterations, check_every, seed, eager, o.autotune != 0, tr, err)) return r;
         S.iterations += tr.halfsteps; S.actions += tr.actions;
         if (int r = set_active(plan, st, all, err)) return r;
         if (!st.rhs64 && o.max_refinements > 0) { if (int r = walloc(st, st.rhs64, st.nnzb * NBLK, err)) return r; }
@@ -880,34 +940,92 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             auto const act = unconverged();
             if (int r = set_active(plan, st, act, err)) return r;
             TfqmrResult t2;
-            if (int r = tfqmr<double>(plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every, seed + k + 1, eager, t2, err)) return r;
+            if (int r = tfqmr<double>(plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every, seed + k + 1, eager, false, t2, err)) return r;
             S.iterations += t2.halfsteps; S.actions += t2.actions; S.refinements++;
             if (int r = set_active(plan, st, act, err)) return r;
             if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
             if (int r = fetch_resid()) return r;
         }
     } else {
+        // FP32 tfQMR inside FP64 refinement, per column group: a group stays in FP32 while each FP32 solve
+        // reduces its true (FP64) residual at least tenfold; a group whose FP32 solve gains less (the FP32
+        // recurrences of an ill-conditioned instance stall) finishes with FP64 solves of its correction.
         if (int r = ensure_work<float>(st, err)) return r;
         if (!st.rhs32) { if (int r = walloc(st, st.rhs32, st.nnzb * NBLK, err)) return r; }
-        double* y64 = reinterpret_cast<double*>(work<float>(st, 0));  // spans work vectors 0 and 1
-        double const floor_rel = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-5;  // gain asked of one FP32 solve
-        std::vector<uint8_t> act = all;
+        double* y64_32 = reinterpret_cast<double*>(work<float>(st, 0));  // spans work vectors 0 and 1
+        double const floor_rel = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;  // gain asked of one FP32 solve
+        int const budget32 = o.fp32_max_iterations > 0 ? o.fp32_max_iterations : o.max_iterations;
+        std::vector<uint8_t> prec64(st.nCols, 0), done(st.nCols, 0);
+        std::vector<double> gres(st.nCols, 1.0);  // max relative residual of each group (||b|| = 1)
+        bool first32 = true;
+        auto group_max = [&](uint32_t c) { double m = 0; for (int j = 0; j < 64; ++j) m = std::max(m, resid[64 * c + j]); return m; };
         for (int k = 0; k <= o.max_refinements; ++k) {
-            if (int r = set_active(plan, st, act, err)) return r;
-            TfqmrResult tr;
-            if (int r = tfqmr<float>(plan, st, k == 0 ? nullptr : st.rhs32, floor_rel, st.tol_abs, o.max_iterations,
-                                     check_every, seed + k, eager, tr, err)) return r;
-            S.iterations += tr.halfsteps; S.actions += tr.actions;
+            std::vector<uint8_t> a32(st.nCols, 0), a64(st.nCols, 0), fresh64(st.nCols, 0);
+            bool any32 = false, any64 = false;
+            for (uint32_t c = 0; c < st.nCols; ++c) {
+                if (done[c]) continue;
+                if (prec64[c]) { a64[c] = 1; any64 = true; } else { a32[c] = 1; any32 = true; }
+            }
+            if (!any32 && !any64) break;
+            if (any32) {
+                if (int r = set_active(plan, st, a32, err)) return r;
+                TfqmrResult tr;
+                if (int r = tfqmr<float>(plan, st, first32 ? nullptr : st.rhs32, floor_rel, st.tol_abs, budget32,
+                                         check_every, seed + k, eager, o.autotune != 0, tr, err)) return r;
+                first32 = false;
+                S.iterations += tr.halfsteps; S.actions += tr.actions;
+                if (int r = set_active(plan, st, a32, err)) return r;
+                if (int r = residual<float>(plan, st, y64_32, st.rhs32, S.actions, err)) return r;
+                if (int r = fetch_resid()) return r;
+                double mx = 0;
+                for (uint32_t c = 0; c < st.nCols; ++c) {
+                    if (!a32[c]) continue;
+                    double const g = group_max(c);
+                    mx = std::max(mx, g);
+                    if (g <= o.tolerance) done[c] = 1;
+                    else if (g > 0.1 * gres[c]) { prec64[c] = 1; fresh64[c] = 1; }
+                    gres[c] = g;
+                }
+                if (o.echo > 0)
+                    std::fprintf(stderr, "# gfa_solve fp32 step %d: %d half-steps, max FP64 residual %.3e\n", k, tr.halfsteps, mx);
+            }
+            bool any_fresh = false;
+            for (uint32_t c = 0; c < st.nCols; ++c) any_fresh |= fresh64[c] != 0;
+            if (any64 || any_fresh) {
+                if (int r = ensure_work<double>(st, err)) return r;
+                if (!st.rhs64) { if (int r = walloc(st, st.rhs64, st.nnzb * NBLK, err)) return r; }
+                double* y64 = work<double>(st, 0);
+                if (any_fresh) {  // the FP64 residual of the groups that leave FP32, as their right-hand side
+                    if (int r = set_active(plan, st, fresh64, err)) return r;
+                    if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
+                }
+                std::vector<uint8_t> b64(st.nCols, 0);
+                for (uint32_t c = 0; c < st.nCols; ++c) b64[c] = (!done[c] && prec64[c]) ? 1 : 0;
+                if (int r = set_active(plan, st, b64, err)) return r;
+                TfqmrResult t2;
+                if (int r = tfqmr<double>(plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every,
+                                          seed + 1000 + k, eager, o.autotune != 0, t2, err)) return r;
+                S.iterations += t2.halfsteps; S.actions += t2.actions;
+                if (int r = set_active(plan, st, b64, err)) return r;
+                if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
+                if (int r = fetch_resid()) return r;
+                double mx = 0;
+                for (uint32_t c = 0; c < st.nCols; ++c) {
+                    if (!b64[c]) continue;
+                    double const g = group_max(c);
+                    mx = std::max(mx, g);
+                    if (g <= o.tolerance) done[c] = 1;
+                    gres[c] = g;
+                }
+                if (o.echo > 0)
+                    std::fprintf(stderr, "# gfa_solve fp64 finish %d: %d half-steps, max residual %.3e\n", k, t2.halfsteps, mx);
+            }
             if (k > 0) S.refinements++;
-            if (int r = set_active(plan, st, act, err)) return r;
-            if (int r = residual<float>(plan, st, y64, st.rhs32, S.actions, err)) return r;
-            if (int r = fetch_resid()) return r;
-            if (o.echo > 0)
-                std::fprintf(stderr, "# gfa_solve fp32 step %d: %d half-steps, max FP64 residual %.3e\n", k, tr.halfsteps, max_resid());
-            if (max_resid() <= o.tolerance) break;
-            act = unconverged();
         }
     }
+    if (o.halfsteps)
+        for (uint32_t q = 0; q < st.ncol; ++q) o.halfsteps[q] = int32_t(st.col_halfsteps[q]);
+    if (o.residuals) std::memcpy(o.residuals, resid.data(), st.ncol * sizeof(double));
     S.max_residual = 0;
     S.converged_columns = 0;
     for (double v : resid) {

@@ -24,7 +24,7 @@ EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS",
 EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
-             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_action_part",
              "gfa_solve", "gfa_last_error"]
 
 
@@ -52,7 +52,8 @@ class SolveOpts(C.Structure):
     _fields_ = [("fp_bits", C.c_int32), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
                 ("max_refinements", C.c_int32), ("check_every", C.c_int32), ("shadow_seed", C.c_uint64),
                 ("X_device", C.c_void_p), ("stream", C.c_void_p), ("inner_tolerance", C.c_double),
-                ("echo", C.c_int32), ("eager", C.c_int32)]
+                ("echo", C.c_int32), ("eager", C.c_int32), ("autotune", C.c_int32),
+                ("fp32_max_iterations", C.c_int32), ("residuals", C.POINTER(C.c_double)), ("halfsteps", C.POINTER(C.c_int32))]
 
 
 class SolveStats(C.Structure):
@@ -88,6 +89,9 @@ def lib():
         L.gfa_action.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_double)]
         L.gfa_action_multiply.argtypes = [vp, C.c_int, vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int,
                                           C.POINTER(C.c_double)]
+        L.gfa_autotune.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
+        L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.gfa_action_part.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gfa_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_double), C.POINTER(SolveStats)]
         L.gfa_last_error.restype = C.c_char_p
         for name in C_SYMBOLS:
@@ -247,8 +251,26 @@ class GreenPlan:
         _check(lib().gfa_action(self.h, fp_bits, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr()), st, C.byref(fl)))
         return fl.value
 
+    def autotune(self, y, x, fp_bits=64, reps=3, stream=None):
+        """Time the action's tile shapes on device tensors x, y (y is overwritten); returns the best ms."""
+        import torch
+        best = C.c_double()
+        st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+        _check(lib().gfa_autotune(self.h, fp_bits, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr()), st, reps,
+                                  C.byref(best)))
+        return best.value
+
+    def multiply_part(self, y, x, part, fp_bits=64, stream=None):
+        """part 1: projection + atom matrices, 2: local kernel (uses the last part-1 coefficients), 3: both."""
+        import torch
+        st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+        _check(lib().gfa_action_part(self.h, fp_bits, part, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr()), st))
+
+    def set_tile_shape(self, fp_bits, columns, lines):
+        _check(lib().gfa_set_tile_shape(self.h, fp_bits, columns, lines))
+
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
-              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False):
+              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0):
         """Solve (H - E S) X = B; returns (rho[nCols, 64] complex, stats dict)."""
         import torch
         o = SolveOpts()
@@ -261,10 +283,19 @@ class GreenPlan:
         o.inner_tolerance = inner_tolerance
         o.echo = echo
         o.eager = 1 if eager else 0
+        o.autotune = 1 if autotune else 0
+        o.fp32_max_iterations = fp32_max_iterations
         o.X_device = C.c_void_p(X.data_ptr()) if X is not None else None
         o.stream = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         rho = np.zeros((self.nCols, 64, 2))
+        res = np.zeros(64 * self.nCols)
+        hst = np.zeros(64 * self.nCols, dtype=np.int32)
+        o.residuals = _dptr(res)
+        o.halfsteps = hst.ctypes.data_as(C.POINTER(C.c_int32))
         st = SolveStats()
         _check(lib().gfa_solve(self.h, C.byref(o), _dptr(rho), C.byref(st)))
         self.refresh_sizes()
-        return rho[..., 0] + 1j * rho[..., 1], st.as_dict()
+        d = st.as_dict()
+        d["column_residuals"] = res
+        d["column_halfsteps"] = hst
+        return rho[..., 0] + 1j * rho[..., 1], d

@@ -1,12 +1,17 @@
 """Benchmark: converged Green-function columns per second at relative residual 1e-9.
 
-Workload (one MI355X): the 32x32x32-block cell (128^3 grid, h = 0.25 Bohr) with 64 seeded Gaussians,
-64 atoms at numax 5, truncation radius 10 Bohr, E = -0.3 + 0.02i Ha, 8th-order kinetic stencil,
-512 source blocks (an 8^3 box at the cell centre, 64 columns each).  A step solves one chunk of
-`--chunk` source blocks (64 * chunk Green-function columns) with the FP32 tfQMR inside FP64
-refinement; every column is verified with the FP64 action and only columns whose true residual
-meets 1e-9 are counted.  Beside it: the plain FP64 solve of the same chunk, the action's roofline from
-HIP events, and the reference's own CPU action (oracle/_ref) timed on the host.
+Workload (one MI355X): the batched energy-contour solve.  The 32x32x32-block cell (128^3 grid, h = 0.25 Bohr)
+with 64 seeded Gaussians and 128 atoms at numax 5; 128 (E, k, radius) instances = 32 contour energies from
+E = -1 Ha to E_F = 0 Ha (Im E 0.005 .. 0.3 Ha) x 4 seeded k-points, truncation radius drawn from
+{6, 8, 10, 12} Bohr per instance; one source block (64 columns) per instance; 8th-order kinetic stencil.
+The cell is wrapped by every radius (2 r <= 32 Bohr), so the k-point leaves the operator unchanged, as in the
+reference.  A step solves the 32 instances of one k-point as one batched plan (column groups = instances)
+with the FP32 tfQMR inside FP64 refinement; every column is verified with the FP64 action and only columns
+whose true residual meets 1e-9 are counted.
+
+Beside it: the plain FP64 path on the same 32 instances (one plan and solve per instance, as the reference's
+contour loop calls its solver per energy), the FP64 batched solve, the local kernel's roofline from HIP events
+on its launch stream, and the reference's own CPU action (oracle/_ref, single thread) on a bounded sample.
 """
 import argparse
 import json
@@ -19,25 +24,59 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BLOCK_BYTES = {64: 2 * 64 * 64 * 8, 32: 2 * 64 * 64 * 4}
+SOURCE = (16, 16, 16)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1, help="batches of one k-point's contour instances")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--chunk", type=int, default=16, help="source blocks per solve")
-    ap.add_argument("--maxit", type=int, default=2000)
+    ap.add_argument("--instances", type=int, default=32, help="instances per batch (<= 32)")
+    ap.add_argument("--maxit", type=int, default=8000)
+    ap.add_argument("--fp32-maxit", type=int, default=1000, help="half-steps allowed to one FP32 solve")
     ap.add_argument("--action-reps", type=int, default=10)
-    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison solve")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-actions", type=int, default=2)
+    ap.add_argument("--cpu-actions", type=int, default=4)
+    ap.add_argument("--mode", choices=["bench", "action"], default="bench",
+                    help="action: only apply the action a few times (for profiler runs)")
     return ap.parse_args()
 
 
-def source_box():
-    off = (32 - 8) // 2
-    return [(x + off, y + off, z + off) for z in range(8) for y in range(8) for x in range(8)]
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def batch(instances, k, n):
+    return instances[32 * (k % 4): 32 * (k % 4) + n]
+
+
+def batch_plan(G, sysm, inst):
+    import numpy as np
+    p = G.GreenPlan(sysm, r_trunc=max(d["radius"] for d in inst), fd_range=(4, 4, 4), sources=[SOURCE] * len(inst),
+                    source_radii=[d["radius"] for d in inst])
+    p.set_inputs(np.array([d["E"] for d in inst]), inst[0]["k"])
+    return p
+
+
+def time_action(plan, fp, part, reps, stream):
+    """Average duration of `reps` applications (part 3: whole action, 2: local kernel) from HIP events recorded
+    on the launch stream."""
+    import torch
+    dt = torch.float64 if fp == 64 else torch.float32
+    x = torch.randn((plan.nnzb, 2, 64, 64), dtype=dt, device="cuda")
+    y = torch.empty_like(x)
+    plan.multiply_part(y, x, 3, fp, stream)   # coefficients for part 2, warm caches of the code objects
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        plan.multiply_part(y, x, part, fp, stream)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    del x, y
+    return ev[0].elapsed_time(ev[1]) / 1e3 / reps
 
 
 def main():
@@ -55,93 +94,98 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    t0 = time.time()
-    sysm = S.wrap_32()
-    E = complex(-0.3, 0.02)
-    srcs = source_box()
-    C = args.chunk
-    nchunks = len(srcs) // C
-    print(f"# system built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
-
-    def plan_for(k):
-        p = G.GreenPlan(sysm, r_trunc=10.0, fd_range=(4, 4, 4), sources=srcs[(k % nchunks) * C:(k % nchunks + 1) * C])
-        p.set_inputs(E)
-        return p
-
     stream = torch.cuda.current_stream()
+    t0 = time.time()
+    sysm = S.contour_32()
+    instances = S.contour_instances()
+    n = min(32, args.instances)
+    log(f"# system built in {time.time() - t0:.1f} s")
 
-    # ---- the measured path: mixed precision solves of chunks ----
+    if args.mode == "action":
+        p = batch_plan(G, sysm, batch(instances, 0, n))
+        for fp in (64, 32):
+            for part in (3, 2):
+                t = time_action(p, fp, part, args.action_reps, stream)
+                log(f"# action fp{fp} part {part}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+        # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
+        x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(2):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"# copy calibration: {x.numel() * 8} bytes read and written per copy")
+        return
+
+    opts = dict(tolerance=1e-9, max_iterations=args.maxit, max_refinements=6, inner_tolerance=1e-4,
+                fp32_max_iterations=args.fp32_maxit)
+    # ---- warmup (also tunes the tile shapes of both precisions) ----
     for w in range(args.warmup):
-        p = plan_for(nchunks - 1 - w)
-        _, st = p.solve(fp_bits=32, tolerance=1e-9, max_iterations=args.maxit, max_refinements=4)
-        print(f"# warmup {w}: {st}", file=sys.stderr, flush=True)
+        p = batch_plan(G, sysm, batch(instances, w, n))
+        _, st = p.solve(fp_bits=32, autotune=True, **opts)
+        log(f"# warmup {w}: {st['converged_columns']}/{st['total_columns']} columns, {st['seconds']:.2f} s")
         p.close()
-    plans = [plan_for(k) for k in range(args.steps)]
+
+    # ---- the measured path: batched mixed-precision solves ----
+    plans = [batch_plan(G, sysm, batch(instances, k, n)) for k in range(args.steps)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     stats = []
     for k in range(args.steps):
-        _, st = plans[k].solve(fp_bits=32, tolerance=1e-9, max_iterations=args.maxit, max_refinements=4)
+        _, st = plans[k].solve(fp_bits=32, autotune=True, **opts)
         stats.append(st)
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t_start
     cols_mixed = sum(s["converged_columns"] for s in stats)
+    per_inst = []
     for k, st in enumerate(stats):
-        print(f"# step {k}: {st}", file=sys.stderr, flush=True)
+        hs = st["column_halfsteps"].reshape(-1, 64).max(axis=1)
+        log(f"# step {k}: {st['converged_columns']}/{st['total_columns']} columns in {st['seconds']:.2f} s, "
+            f"{st['iterations']} half-steps, max residual {st['max_residual']:.2e}")
+        if k == 0:
+            per_inst = [int(h) for h in hs]
     nnzb = plans[0].nnzb
     for p in plans[1:]:
         p.close()
     plan = plans[0]
+    first = batch(instances, 0, n)
 
-    # ---- action roofline from HIP events on the launch stream ----
-    xd = torch.randn((nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
-    yd = torch.empty_like(xd)
-    plan.multiply(yd, xd, 64)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ev[0].record(stream)
-    for _ in range(args.action_reps):
-        plan.multiply(yd, xd, 64)
-    ev[1].record(stream)
-    torch.cuda.synchronize()
-    t_act64 = ev[0].elapsed_time(ev[1]) / 1e3 / args.action_reps
-    x32, y32 = xd.float(), yd.float()
-    del xd, yd
-    plan.multiply(y32, x32, 32)
-    ev[0].record(stream)
-    for _ in range(args.action_reps):
-        plan.multiply(y32, x32, 32)
-    ev[1].record(stream)
-    torch.cuda.synchronize()
-    t_act32 = ev[0].elapsed_time(ev[1]) / 1e3 / args.action_reps
-    del x32, y32
-    torch.cuda.empty_cache()
-    nrows = plan.nRows
-    bytes64 = nnzb * 2 * (2 * 64 * 64 * 8) + nrows * 64 * 8
-    bytes32 = nnzb * 2 * (2 * 64 * 64 * 4) + nrows * 64 * 8
-
-    # ---- plain FP64 solve of the same first chunk ----
-    t_fp64 = None
-    cols_fp64 = 0
-    st64 = None
+    # ---- FP64: the same instances batched, and one at a time (the plain drop-in path) ----
+    fp64 = {}
     if not args.no_fp64:
-        plan.close()
-        plan = plan_for(0)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        _, st64 = plan.solve(fp_bits=64, tolerance=1e-9, max_iterations=args.maxit, max_refinements=0)
-        torch.cuda.synchronize()
-        t_fp64 = time.perf_counter() - t1
-        cols_fp64 = st64["converged_columns"]
-        print(f"# fp64: {st64} in {t_fp64:.3f} s", file=sys.stderr, flush=True)
+        _, s64 = plan.solve(fp_bits=64, autotune=True, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2)
+        t_b64 = time.perf_counter() - t1
+        log(f"# fp64 batched: {s64['converged_columns']} columns in {t_b64:.2f} s")
+        fp64["batched_columns_per_s"] = round(s64["converged_columns"] / t_b64, 3)
+        fp64["batched_half_steps"] = [int(h) for h in s64["column_halfsteps"].reshape(-1, 64).max(axis=1)]
+        cols, t_seq = 0, 0.0
+        for d in first:   # plan built untimed (as the batched plans), device upload and solve timed
+            p1 = batch_plan(G, sysm, [d])
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _, s1 = p1.solve(fp_bits=64, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2)
+            t_seq += time.perf_counter() - t1
+            cols += s1["converged_columns"]
+            p1.close()
+        log(f"# fp64 one instance at a time: {cols} columns in {t_seq:.2f} s")
+        fp64.update(columns_per_s=round(cols / t_seq, 3), seconds=round(t_seq, 3), converged_columns=cols,
+                    actions=int(s64["actions"]))
+
+    # ---- roofline of the local kernel (potential + kinetic + SHO add-back), HIP events on its stream ----
+    t_loc64 = time_action(plan, 64, 2, args.action_reps, stream)
+    t_loc32 = time_action(plan, 32, 2, args.action_reps, stream)
+    t_act64 = time_action(plan, 64, 3, args.action_reps, stream)
+    t_act32 = time_action(plan, 32, 3, args.action_reps, stream)
+    nrows = plan.nRows
+    alg64 = nnzb * 2 * BLOCK_BYTES[64] + nrows * 64 * 8
+    alg32 = nnzb * 2 * BLOCK_BYTES[32] + nrows * 64 * 8
     plan.close()
 
     value = cols_mixed / t_mixed
-    fp64_rate = cols_fp64 / t_fp64 if t_fp64 else None
-
-    # ---- CPU baseline: the reference's CPU action on one source block of the same workload ----
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sysm, E, args, st64)
+        cpu = cpu_baseline(sysm, first, per_inst, fp64.get("batched_half_steps"), args)
 
     out = {
         "metric": "Gf columns solved/s at resid 1e-9 (mixed+batched vs FP64); action % HBM peak",
@@ -154,71 +198,80 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c32 (FP32 tfQMR) + f64 refinement",
-        "data": "synthetic (seeded Gaussians potential, random atom matrices)",
-        "config": {"workload": "32^3-block cell, 64 atoms numax 5, r_trunc 10 Bohr, E=-0.3+0.02i, 8th-order FD",
-                   "source_blocks_per_step": C, "columns_per_step": 64 * C, "nnzb_per_step": int(nnzb),
-                   "mode": "mixed FP32 tfQMR + FP64 refinement (batched persistent launch not yet built)"},
-        "fp64": {"columns_per_s": None if fp64_rate is None else round(fp64_rate, 3),
-                 "seconds": t_fp64, "half_steps": None if st64 is None else st64["iterations"],
-                 "converged_columns": cols_fp64},
-        "mixed_over_fp64": None if not fp64_rate else round(value / fp64_rate, 3),
-        "mixed_stats": stats,
+        "dtype": "c32 tfQMR + f64 refinement (per-group FP64 finish)",
+        "data": "synthetic (seeded Gaussian potential, random atom matrices, Gauss-Legendre contour)",
+        "config": {"workload": "batched contour: 32^3-block cell, 128 atoms numax 5, 32 energies -1..0 Ha "
+                               "(Im 0.005..0.3) x 4 k-points, radii {6,8,10,12} Bohr, one source block each",
+                   "instances_per_step": n, "columns_per_step": 64 * n, "nnzb_per_step": int(nnzb)},
+        "mixed_half_steps_per_instance": per_inst,
+        "fp64": fp64 or None,
+        "mixed_over_fp64": round(value / fp64["columns_per_s"], 3) if fp64 else None,
+        "mixed_over_fp64_batched": round(value / fp64["batched_columns_per_s"], 3) if fp64 else None,
         "action": {"fp64_ms": round(t_act64 * 1e3, 3), "fp32_ms": round(t_act32 * 1e3, 3),
-                   "fp64_GBs": round(bytes64 / t_act64 / 1e9, 1), "fp32_GBs": round(bytes32 / t_act32 / 1e9, 1)},
-        "roofline": {"bound": "hbm", "achieved": round(bytes64 / t_act64 / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(bytes64 / t_act64 / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": "FP64 action (k_sho_project + k_sho_atoms + k_local), algorithmic bytes = "
-                               "nnzb*2*64KiB + nRows*512B"},
+                   "local_fp64_ms": round(t_loc64 * 1e3, 3), "local_fp32_ms": round(t_loc32 * 1e3, 3),
+                   "fp64_GBs": round(alg64 / t_act64 / 1e9, 1), "fp32_GBs": round(alg32 / t_act32 / 1e9, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(alg64 / t_loc64 / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg64 / t_loc64 / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_local FP64 (potential + 8th-order kinetic + SHO add-back); algorithmic bytes "
+                               "per launch = nnzb * 2 * 64 KiB + nRows * 512 B"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
-def cpu_baseline(sysm, E, args, st64):
-    """The reference's own CPU action (compiled from its sources into oracle/_ref) on one source block
-    of the workload, extrapolated to columns/s with the FP64 solve's action count.  Falls back to the
-    oracle's C restatement when the reference build is absent."""
+def cpu_baseline(sysm, first, half_steps_mixed, half_steps_fp64, args):
+    """The reference's own CPU action (built from its sources into oracle/_ref; the oracle's C restatement if
+    that build is absent), single thread, on one radius-8 instance; scaled by block count to every instance
+    and by the FP64 solve's actions per instance to converged columns per second (vector updates of the CPU
+    solve not counted, so this is an upper bound of the CPU rate)."""
     import numpy as np
+    from angstromcube_amd import gfa as G
     from oracle import refgreen as R
+    nblocks = {}
+    for d in first:
+        r = d["radius"]
+        if r not in nblocks:
+            p = G.GreenPlan(sysm, r_trunc=r, fd_range=(4, 4, 4), sources=[SOURCE])
+            nblocks[r] = p.nnzb
+            p.close()
+    sample = next(d for d in first if d["radius"] == 8.0) if any(d["radius"] == 8.0 for d in first) else first[0]
     kind = "reference"
-    src = source_box()[0]
     x = np.random.default_rng(3).standard_normal
     try:
         if not R.available():
             raise RuntimeError("no reference build")
-        rp = R.RefPlan(sysm, r_trunc=10.0, fd_range=(4, 4, 4), nsrc=(1, 1, 1))
-        rp.set_inputs(E)
-        n = rp.nnzb
-        xv = x((n, 2, 64, 64))
+        rp = R.RefPlan(sysm, r_trunc=sample["radius"], fd_range=(4, 4, 4), nsrc=(1, 1, 1))
+        rp.set_inputs(sample["E"])
+        nb = rp.nnzb
+        xv = x((nb, 2, 64, 64))
         t = time.perf_counter()
         for _ in range(args.cpu_actions):
             rp.action(xv)
         t_action = (time.perf_counter() - t) / args.cpu_actions
-    except Exception as e:  # the oracle's restatement of the same action
-        print(f"# cpu baseline falls back to the oracle: {e}", file=sys.stderr)
+    except Exception as e:
+        log(f"# cpu baseline falls back to the oracle: {e}")
         kind = "port"
-        from angstromcube_amd import gfa as G
         from oracle import oracle as O
-        p = G.GreenPlan(sysm, r_trunc=10.0, fd_range=(4, 4, 4), sources=[src])
-        o = O.Oracle(p, E)
-        n = p.nnzb
-        xv = x((n, 2, 64, 64))
+        p = G.GreenPlan(sysm, r_trunc=sample["radius"], fd_range=(4, 4, 4), sources=[SOURCE])
+        o = O.Oracle(p, sample["E"])
+        nb = p.nnzb
+        xv = x((nb, 2, 64, 64))
         t = time.perf_counter()
         for _ in range(args.cpu_actions):
             o.action(xv)
         t_action = (time.perf_counter() - t) / args.cpu_actions
         p.close()
-    actions_per_solve = None if st64 is None else st64["actions"]
-    if not actions_per_solve:
+    per_block = t_action / nb
+    hs = half_steps_fp64 or half_steps_mixed
+    if not hs:
         return {"value": None, "unit": "Gf columns/s", "cores": 1, "kind": kind,
-                "sample": f"{args.cpu_actions} CPU actions, {t_action:.3f} s each; no FP64 action count"}
-    t_solve = t_action * actions_per_solve
-    return {"value": round(64.0 / t_solve, 6), "unit": "Gf columns/s", "cores": 1, "kind": kind,
-            "sample": f"{args.cpu_actions} single-thread reference CPU actions on one source block "
-                      f"({n} blocks), {t_action:.2f} s each, times the {actions_per_solve} actions of the "
-                      f"FP64 GPU solve per chunk (vector updates not counted); host {platform.processor() or platform.machine()}"}
+                "sample": f"{args.cpu_actions} CPU actions of {t_action:.2f} s; no iteration counts"}
+    t_cpu = sum(per_block * nblocks[d["radius"]] * (h + 2) for d, h in zip(first, hs))
+    return {"value": round(64.0 * len(first) / t_cpu, 6), "unit": "Gf columns/s", "cores": 1, "kind": kind,
+            "sample": f"{args.cpu_actions} single-thread CPU actions on the radius-{sample['radius']:g} instance "
+                      f"({nb} blocks, {t_action:.2f} s each), scaled by block count to the {len(first)} instances "
+                      f"and by the FP64 GPU solve's actions per instance; vector updates not counted; "
+                      f"host {platform.processor() or platform.machine()}"}
 
 
 if __name__ == "__main__":

@@ -137,6 +137,20 @@ int gfa_action_multiply(gfa_plan* plan, int fp_bits, void* y, const void* x, con
                         uint32_t nnzb, uint32_t nCols, uint32_t l2nX, void* stream, int precondition,
                         double* nflops);
 
+/* ---- one part of the action, for timing and tests: part 1 = SHO projection and atom-matrix multiply (into the
+ *      plan's coefficient workspace), 2 = the local kernel (potential, kinetic and SHO add-back of the workspace
+ *      coefficients, writes y), 3 = both (= gfa_action).  Part 2 alone uses the coefficients of the last part 1
+ *      of the same precision.  Mirrors the reference's separate multiply of each term
+ *      (green_potential / green_kinetic / green_dyadic::multiply, include/green_action.hxx:154-175). ---- */
+int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* x, void* stream);
+
+/* ---- tile-shape autotuner: times every tile shape of the action on the caller's device vectors x, y (the
+ *      layout of gfa_action; y is overwritten), reps applications each, and keeps the fastest for fp_bits.
+ *      No counterpart in the reference, whose kernels have fixed launch shapes. ---- */
+int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, int reps, double* best_ms);
+/* pins the tile shape instead: columns per work-group 16 or 32, kinetic lines per thread in flight 1, 2 or 4 */
+int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int columns, int lines);
+
 /* ---- the solve (H - E S) X = B with B = unit columns on the diagonal blocks; replaces
  *      green_solver_t::solve / action_t::solve (include/green_solver.hxx:24-30, include/green_action.hxx:181-241)
  *      and tfqmrgpu::solve(action, buffer, 1e-9, maxiter, 0, true). ---- */
@@ -149,9 +163,14 @@ typedef struct {
     uint64_t shadow_seed;       /* seed of the random shadow residual                                 */
     void* X_device;             /* optional: receives the FP64 solution [nnzb][2][64][64] (device)   */
     void* stream;               /* hipStream_t or NULL                                                */
-    double inner_tolerance;     /* fp_bits 32: residual reduction asked of one FP32 solve (0: 1e-5)   */
+    double inner_tolerance;     /* fp_bits 32: residual reduction asked of one FP32 solve (0: 1e-4)   */
     int32_t echo;               /* >0: one line per inner solve on stderr                             */
     int32_t eager;              /* 0: replay each iteration from a captured HIP graph; 1: plain launches */
+    int32_t autotune;           /* 1: time the action's tile shapes on the solver's vectors first (once per precision) */
+    int32_t fp32_max_iterations;/* fp_bits 32: half-steps allowed to one FP32 solve (0: max_iterations); a column
+                                 * group whose FP32 solve reduces its residual less than tenfold continues in FP64 */
+    double* residuals;          /* optional host output [64 nCols]: final relative residual of every column          */
+    int32_t* halfsteps;         /* optional host output [64 nCols]: tfQMR half-steps of every column (all solves)     */
 } gfa_solve_opts;
 
 typedef struct {

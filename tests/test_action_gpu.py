@@ -84,3 +84,59 @@ def test_action_rejects_bad_arguments(device):
         plan.multiply(yd, xd, fp_bits=16)
     with pytest.raises(G.GfaError, match="in place"):
         G._check(G.lib().gfa_action(plan.h, 64, G.C.c_void_p(xd.data_ptr()), G.C.c_void_p(xd.data_ptr()), None, None))
+
+
+@pytest.mark.parametrize("columns,lines", [(16, 1), (16, 2), (16, 4), (32, 1), (32, 2), (32, 4)])
+def test_every_tile_shape_matches_oracle(device, columns, lines):
+    """Each tile shape the autotuner may pick gives the oracle's action (FP64 1e-12, FP32 2e-5), with periodic
+    images (8th order) and with the 16th-order stencil."""
+    import torch
+    for name, mk, kw, E, k in [c for c in cases(small_only=True) if c[0] in ("periodic8_k", "isolated6_fd16")]:
+        plan = G.GreenPlan(mk(), **kw)
+        plan.set_inputs(E, k)
+        x = _rand((plan.nnzb, 2, 64, 64), 21)
+        y_ref = O.Oracle(plan, E, k).action(x)
+        for fp, tol, dt in ((64, 1e-12, torch.float64), (32, 2e-5, torch.float32)):
+            plan.set_tile_shape(fp, columns, lines)
+            xd = torch.from_numpy(x).to(device=device, dtype=dt)
+            yd = torch.empty_like(xd)
+            print(f"tile {columns}x{lines} {name} fp{fp}", flush=True)
+            plan.multiply(yd, xd, fp_bits=fp)
+            torch.cuda.synchronize()
+            err = np.abs(yd.double().cpu().numpy() - y_ref).max() / np.abs(y_ref).max()
+            assert err < tol, f"{name} fp{fp} tile {columns}x{lines}: {err:.3e}"
+
+
+def test_autotune_keeps_the_action_exact(device):
+    import torch
+    name, mk, kw, E, k = cases(small_only=True)[1]
+    plan = G.GreenPlan(mk(), **kw)
+    plan.set_inputs(E, k)
+    x = _rand((plan.nnzb, 2, 64, 64), 22)
+    xd = torch.from_numpy(x).to(device)
+    yd = torch.empty_like(xd)
+    best = plan.autotune(yd, xd, fp_bits=64, reps=2)
+    assert best > 0
+    plan.multiply(yd, xd, fp_bits=64)
+    torch.cuda.synchronize()
+    y_ref = O.Oracle(plan, E, k).action(x)
+    assert np.abs(yd.cpu().numpy() - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+    with pytest.raises(G.GfaError, match="tile"):
+        plan.set_tile_shape(64, 24, 2)
+
+
+def test_action_parts_compose(device):
+    """Projection (part 1) followed by the local kernel (part 2) is the whole action, bit for bit."""
+    import torch
+    name, mk, kw, E, k = cases(small_only=True)[1]
+    plan = G.GreenPlan(mk(), **kw)
+    plan.set_inputs(E, k)
+    xd = torch.from_numpy(_rand((plan.nnzb, 2, 64, 64), 23)).to(device)
+    y_all, y_parts = torch.empty_like(xd), torch.empty_like(xd)
+    plan.multiply(y_all, xd, fp_bits=64)
+    plan.multiply_part(y_parts, xd, 1, fp_bits=64)
+    plan.multiply_part(y_parts, xd, 2, fp_bits=64)
+    torch.cuda.synchronize()
+    assert torch.equal(y_all, y_parts)
+    with pytest.raises(G.GfaError, match="part"):
+        plan.multiply_part(y_parts, xd, 4, fp_bits=64)
