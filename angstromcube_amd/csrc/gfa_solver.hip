@@ -361,6 +361,22 @@ __global__ __launch_bounds__(256) void k_vnext(T* __restrict__ v, T const* __res
     store_partials<T, 2>(acc, g.segpart, s);
 }
 
+// x = 0 over the blocks of the active segments
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_seg(double* __restrict__ x, SegArgs g) {
+    SEG_PROLOGUE
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            double z[V];
+#pragma unroll
+            for (int t = 0; t < V; ++t) z[t] = 0;
+            std_<V>(x + OFF(0, i), z); std_<V>(x + OFF(1, i), z);
+        }
+    }
+}
+
 // r = b - y (FP64, b = unit columns on the diagonal blocks); partial |r|^2; optionally r_out = r
 template <typename T>
 __global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, uint8_t const* __restrict__ diag,
@@ -949,7 +965,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     } else {
         // FP32 tfQMR inside FP64 refinement, per column group: a group stays in FP32 while each FP32 solve
         // reduces its true (FP64) residual at least tenfold; a group whose FP32 solve gains less (the FP32
-        // recurrences of an ill-conditioned instance stall) finishes with FP64 solves of its correction.
+        // recurrences of an ill-conditioned instance stall) is solved again in FP64 from zero.
         if (int r = ensure_work<float>(st, err)) return r;
         if (!st.rhs32) { if (int r = walloc(st, st.rhs32, st.nnzb * NBLK, err)) return r; }
         double* y64_32 = reinterpret_cast<double*>(work<float>(st, 0));  // spans work vectors 0 and 1
@@ -995,8 +1011,13 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
                 if (int r = ensure_work<double>(st, err)) return r;
                 if (!st.rhs64) { if (int r = walloc(st, st.rhs64, st.nnzb * NBLK, err)) return r; }
                 double* y64 = work<double>(st, 0);
-                if (any_fresh) {  // the FP64 residual of the groups that leave FP32, as their right-hand side
+                if (any_fresh) {
+                    // groups leaving FP32 start over in FP64 from x = 0 (continuing from the stalled FP32
+                    // iterate converges more slowly than a fresh FP64 solve): rhs = b - A 0 = b
                     if (int r = set_active(plan, st, fresh64, err)) return r;
+                    SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+                    hipLaunchKernelGGL((k_zero_seg<double>), dim3(st.nseg), dim3(256), 0, s, st.x64, g);
+                    HIP_TRY(hipGetLastError());
                     if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
                 }
                 std::vector<uint8_t> b64(st.nCols, 0);

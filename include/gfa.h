@@ -168,7 +168,8 @@ typedef struct {
     int32_t eager;              /* 0: replay each iteration from a captured HIP graph; 1: plain launches */
     int32_t autotune;           /* 1: time the action's tile shapes on the solver's vectors first (once per precision) */
     int32_t fp32_max_iterations;/* fp_bits 32: half-steps allowed to one FP32 solve (0: max_iterations); a column
-                                 * group whose FP32 solve reduces its residual less than tenfold continues in FP64 */
+                                 * group whose FP32 solve reduces its residual less than tenfold is solved again
+                                 * in FP64 from zero                                                          */
     double* residuals;          /* optional host output [64 nCols]: final relative residual of every column          */
     int32_t* halfsteps;         /* optional host output [64 nCols]: tfQMR half-steps of every column (all solves)     */
 } gfa_solve_opts;
