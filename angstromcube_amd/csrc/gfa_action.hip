@@ -28,6 +28,9 @@ namespace gfa {
 namespace {
 
 constexpr int NBLK = 2 * 64 * 64;  // elements of one block
+#ifndef LOCAL_TILE
+#define LOCAL_TILE 8   // xy tile edge (blocks) of the local kernel's processing order
+#endif
 
 #define HIP_TRY(call)                                                              \
     do {                                                                           \
@@ -332,9 +335,11 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
 
     uint64_t lin = blockIdx.x;
     if (a.per_xcd) lin = uint64_t(blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
-    uint64_t const item = lin / NSLICE;
-    int const slice = int(lin % NSLICE);
-    if (item >= a.nitems) return;  // uniform over the work-group
+    // slice-major: all blocks of column slice 0 in processing order, then slice 1, ...; the neighbour slices
+    // a work-group reads were then read shortly before by work-groups of the same XCD
+    uint64_t const item = lin % a.nitems;
+    int const slice = int(lin / a.nitems);
+    if (slice >= NSLICE) return;  // uniform over the work-group
     uint64_t const blk = a.order[item];
     int const tid = threadIdx.x;
 
@@ -488,6 +493,262 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
 }
 
 
+// SHO add-back of one atom image into 16 register accumulators (z plane zq of one column and plane); the same
+// sum-factorised contraction as sho_add_atom.
+template <typename real_t>
+__device__ __forceinline__ void sho_add_regs(real_t (&out)[16], real_t const (*H)[3][4], float const (*xi2)[4],
+                                             float R2, real_t const* __restrict__ dsrc, size_t dstride, int zq, int L) {
+    int s = 0;
+    real_t acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = 0;
+#pragma unroll 1
+    for (int iz = 0; iz <= L; ++iz) {
+        real_t byx[4][4];
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int xx = 0; xx < 4; ++xx) byx[y][xx] = 0;
+#pragma unroll 1
+        for (int iy = 0; iy <= L - iz; ++iy) {
+            real_t ax[4] = {0, 0, 0, 0};
+#pragma unroll 1
+            for (int ix = 0; ix <= L - iz - iy; ++ix) {
+                real_t const ca = dsrc[size_t(s) * dstride];
+                ++s;
+#pragma unroll
+                for (int xx = 0; xx < 4; ++xx) ax[xx] += H[ix][0][xx] * ca;
+            }
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                real_t const hy_ = H[iy][1][y];
+#pragma unroll
+                for (int xx = 0; xx < 4; ++xx) byx[y][xx] += hy_ * ax[xx];
+            }
+        }
+        real_t const hz_ = H[iz][2][zq];
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int xx = 0; xx < 4; ++xx) acc[y * 4 + xx] += hz_ * byx[y][xx];
+    }
+    float const d2z = xi2[2][zq] - R2;
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        float const d2yz = xi2[1][y] + d2z;
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx) {
+            float const d2 = xi2[0][xx] + d2yz;
+            bool const in = (d2z < 0) && (d2yz < 0) && (d2 < 0);
+            out[y * 4 + xx] += in ? acc[y * 4 + xx] : real_t(0);
+        }
+    }
+}
+
+// value of a neighbour block at element offset `off` (this thread's plane) with the Bloch phase of a periodic
+// image applied: f x_p - (+/-) g x_other; off_o is the offset of the other plane
+template <typename real_t, bool IMG>
+__device__ __forceinline__ real_t nb_value(real_t const* __restrict__ b, size_t off, size_t off_o, real_t fr, real_t fi) {
+    real_t v = b[off];
+    if (IMG) v = fr * v - fi * b[off_o];
+    return v;
+}
+
+// the finite-difference weights applied to a window w of 4 (2 NH + 1) values centred at c
+template <typename real_t, int NH, int NW>
+__device__ __forceinline__ real_t stencil(LocalArgs<real_t> const& a, int d, real_t const (&w)[NW], int c) {
+    real_t t;
+    if (NH == 2 && a.nh[d] == 2) {
+        t = a.c16[d][0] * w[c];
+#pragma unroll
+        for (int m = 1; m <= 8; ++m) {
+            int const lo = c - m, hi = c + m;
+            t += a.c16[d][m] * (w[lo < 0 ? 0 : lo] + w[hi >= NW ? NW - 1 : hi]);
+        }
+    } else {
+        t = a.c8[d][0] * w[c];
+#pragma unroll
+        for (int m = 1; m <= 4; ++m) t += a.c8[d][m] * (w[c - m] + w[c + m]);
+    }
+    return t;
+}
+
+// z-direction stencil for the thread's plane ZQ: window positions z' = ZQ - 4 NH .. ZQ + 4 NH along z at every
+// (y, x); inside the block from LDS (own plane from registers), outside from the neighbour blocks
+template <typename real_t, int NH, bool IMG, int JW, int ZQ>
+__device__ __forceinline__ void z_pass(LocalArgs<real_t> const& a, real_t (&acc)[16], real_t const (&own)[16],
+                                       real_t const* __restrict__ xsp, real_t const* const (&nb)[2][NH],
+                                       real_t const (&fr)[2][NH], real_t const (&fi)[2][NH], size_t off_p, size_t off_o) {
+    constexpr int NWZ = 8 * NH + 1;   // z' = ZQ - 4 NH .. ZQ + 4 NH
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        real_t w[NWZ];
+#pragma unroll
+        for (int m = 0; m < NWZ; ++m) {
+            int const z = ZQ - 4 * NH + m;   // z' relative to this block
+            if (z == ZQ) w[m] = own[k];
+            else if (z >= 0 && z < 4) w[m] = xsp[(z * 16 + k) * JW];
+            else if (z < 0) {
+                int const kb = (-z - 1) / 4, zz = z + 4 * (kb + 1);
+                w[m] = nb_value<real_t, IMG>(nb[0][kb], off_p + size_t(zz * 16 + k) * 64, off_o + size_t(zz * 16 + k) * 64,
+                                             fr[0][kb], fi[0][kb]);
+            } else {
+                int const kb = (z - 4) / 4, zz = z - 4 * (kb + 1);
+                w[m] = nb_value<real_t, IMG>(nb[1][kb], off_p + size_t(zz * 16 + k) * 64, off_o + size_t(zz * 16 + k) * 64,
+                                             fr[1][kb], fi[1][kb]);
+            }
+        }
+        acc[k] += stencil<real_t, NH, NWZ>(a, 2, w, 4 * NH);
+    }
+}
+
+// Register-plane variant of the local kernel: one work-group per (block, JW-column slice), 8 JW threads; thread
+// (jl, p, zq) owns the 16 points of z plane zq of column j = slice JW + jl, plane p, and accumulates potential,
+// kinetic x, y, z and the SHO add-back in registers.  LDS holds the block slice only (read by the z-stencil and
+// the imaginary part of the potential).  zq is uniform within a wave (2 JW >= 64), so the z-stencil is
+// specialised per plane.
+template <typename real_t, int NH, bool IMG, int JW>
+__global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
+    static_assert(2 * JW >= 64, "zq must be uniform within a wave");
+    constexpr int T = 8 * JW;
+    constexpr int NW = 4 * (2 * NH + 1);
+    constexpr int NSLICE = 64 / JW;
+    __shared__ real_t xs[2 * 64 * JW];
+    __shared__ real_t H[8][3][4];
+    __shared__ float xi2[3][4];
+    __shared__ double apos[4];
+    __shared__ float cpos[4];
+    __shared__ double hgrid[4];
+
+    uint64_t lin = blockIdx.x;
+    if (a.per_xcd) lin = uint64_t(blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
+    // slice-major: all blocks of column slice 0 in processing order, then slice 1, ...; the neighbour slices
+    // a work-group reads were then read shortly before by work-groups of the same XCD
+    uint64_t const item = lin % a.nitems;
+    int const slice = int(lin / a.nitems);
+    if (slice >= NSLICE) return;  // uniform over the work-group
+    uint64_t const blk = a.order[item];
+    int const tid = threadIdx.x;
+
+    using vec_t = typename std::conditional<sizeof(real_t) == 8, double2, float4>::type;
+    constexpr int VE = 16 / sizeof(real_t), VPR = JW / VE;
+    {
+        real_t const* src = a.x + blk * NBLK + slice * JW;
+#pragma unroll
+        for (int k = 0; k < 2 * 64 * VPR / T; ++k) {
+            int const v = k * T + tid, row = v / VPR, cv = v % VPR;
+            *reinterpret_cast<vec_t*>(xs + row * JW + cv * VE) = *reinterpret_cast<vec_t const*>(src + row * 64 + cv * VE);
+        }
+    }
+    if (tid < 4) hgrid[tid] = a.hgrid[tid];
+    __syncthreads();
+
+    int const jl = tid % JW, p = (tid / JW) & 1, zq = tid / (2 * JW);
+    int const j = slice * JW + jl;
+    real_t const* xsp = xs + p * 64 * JW + jl;
+    real_t const* xso = xs + (1 - p) * 64 * JW + jl;
+    size_t const off_p = size_t(p) * 4096 + j, off_o = size_t(1 - p) * 4096 + j;
+
+    real_t own[16], acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) own[k] = xsp[(zq * 16 + k) * JW];
+
+    // ---- potential: (V - E) x, the imaginary part of E coupling the two planes ----
+    {
+        int32_t const iv = a.veff[blk];
+        real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
+        double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + zq * 16;
+        uint32_t const colE = a.colindx[blk];
+        real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
+        real_t const Vim = E_im * real_t(1 - 2 * p);
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            acc[k] = (real_t(Vb[k]) * vm - E_re) * own[k] + Vim * xso[(zq * 16 + k) * JW];
+    }
+
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        real_t const* nb[2][NH];
+        real_t fr[2][NH], fi[2][NH];
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+                uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
+                nb[sd][k] = (code == -1) ? a.zero : a.x + idx * NBLK;
+                bool const img = IMG && (code <= -2);
+                fr[sd][k] = img ? real_t(a.phase[d][sd][0]) : real_t(1);
+                fi[sd][k] = img ? real_t(a.phase[d][sd][1]) * real_t(1 - 2 * p) : real_t(0);
+            }
+        if (d < 2) {
+            // lines of this plane: along x (d = 0) the rows y, along y (d = 1) the columns x
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                real_t w[NW];   // the line through this block and NH neighbour blocks on each side
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);   // point of the plane on the line
+                    size_t const e = size_t(zq * 16 + k) * 64;
+#pragma unroll
+                    for (int kk = 0; kk < NH; ++kk) {
+                        w[4 * (NH - 1 - kk) + q] = nb_value<real_t, IMG>(nb[0][kk], off_p + e, off_o + e, fr[0][kk], fi[0][kk]);
+                        w[4 * (NH + 1 + kk) + q] = nb_value<real_t, IMG>(nb[1][kk], off_p + e, off_o + e, fr[1][kk], fi[1][kk]);
+                    }
+                    w[4 * NH + q] = own[k];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);
+                    acc[k] += stencil<real_t, NH, NW>(a, d, w, 4 * NH + q);
+                }
+            }
+        } else {
+            switch (zq) {
+                case 0: z_pass<real_t, NH, IMG, JW, 0>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
+                case 1: z_pass<real_t, NH, IMG, JW, 1>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
+                case 2: z_pass<real_t, NH, IMG, JW, 2>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
+                default: z_pass<real_t, NH, IMG, JW, 3>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
+            }
+        }
+    }
+
+    // ---- SHO add-back of the atom images (numax <= 5) touching this block; the coefficients of each image are
+    //      staged in the LDS buffer of the block slice, which is free after the kinetic passes ----
+    uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
+    if (b1 > b0) {
+        uint32_t const col = a.colindx[blk];
+        float const R2 = float(a.hgrid[3] * a.hgrid[3]);
+        real_t* cs = xs;    // [nc][2][JW]
+        for (uint32_t b = b0; b < b1; ++b) {
+            uint32_t const img = a.add_index[b];
+            int const L = a.image_lmax[img];
+            if (L > 5) continue;  // done by k_sho_add_wide
+            int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
+            __syncthreads();
+            if (tid < 4) apos[tid] = a.image_pos[4 * size_t(img) + tid];
+            if (tid < 3) cpos[tid] = a.row_pos[4 * size_t(a.rowindx[blk]) + tid];
+            {
+                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
+                for (int v = tid; v < 2 * nc * VPR; v += T) {
+                    int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
+                    *reinterpret_cast<vec_t*>(cs + row * JW + cv * VE) =
+                        *reinterpret_cast<vec_t const*>(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * VE);
+                }
+            }
+            __syncthreads();
+            if (tid < 12) hermite_1d<real_t>(H, xi2, tid, L, apos, cpos, hgrid);
+            __syncthreads();
+            sho_add_regs<real_t>(acc, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
+        }
+    }
+
+    real_t* dst = a.y + blk * NBLK + off_p + size_t(zq * 16) * 64;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) dst[size_t(k) * 64] = acc[k];
+}
+
+
 // SHO add-back of atom images with numax 6 or 7, y += P d, for the (block, image) pairs listed in
 // pairs[]: one work-group of 256 threads = (column j, plane p, z plane zq) per pair.  Runs after k_local.
 template <typename real_t>
@@ -583,10 +844,10 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
     if ((s = dupload(d, d.rowindx, p.rowindx, err))) return s;
     if ((s = dupload(d, d.veff, p.veff_index, err))) return s;
     if ((s = dupload(d, d.row_pos, p.row_pos, err))) return s;
-    {   // processing order of k_local: per source column, xy tiles of TILE x TILE target blocks swept
+    {   // processing order of the local kernel: per source column, xy tiles of TILE x TILE target blocks swept
         // along z, so that the 6 stencil neighbours of a block were read shortly before (z neighbours
         // TILE^2 blocks back); consecutive work items then share one XCD's L2
-        int const TILE = 4;
+        int const TILE = LOCAL_TILE;
         std::vector<uint64_t> key(p.nnzb);
         int32_t lo[3] = {1 << 30, 1 << 30, 1 << 30};
         for (uint32_t r = 0; r < p.nRows; ++r)
@@ -717,16 +978,21 @@ int device_upload_values(Plan const& p, DevicePlan& d, std::string& err) {
     return 0;
 }
 
-// k_local tile shapes: JW columns per work-group (16 or 32), LB kinetic lines per thread in flight (1, 2, 4)
+// Local-kernel tile shapes.  kind 0: k_local (LDS accumulator), JW columns per work-group (16 or 32), LB
+// kinetic lines per thread in flight (1, 2, 4).  kind 1: k_plane (register accumulator), JW = 32 or 64.
 template <typename real_t, int NH, bool IMG>
-static void launch_local(int jw, int lb, dim3 g, hipStream_t stream, LocalArgs<real_t> const& a) {
+static void launch_local(int kind, int jw, int lb, dim3 g, hipStream_t stream, LocalArgs<real_t> const& a) {
 #define KL(JW_, LB_) hipLaunchKernelGGL((k_local<real_t, NH, IMG, JW_, LB_>), g, dim3(8 * JW_), 0, stream, a)
-    if (jw == 16) {
+#define KP(JW_) hipLaunchKernelGGL((k_plane<real_t, NH, IMG, JW_>), g, dim3(8 * JW_), 0, stream, a)
+    if (kind == 1) {
+        if (jw == 64) KP(64); else KP(32);
+    } else if (jw == 16) {
         if (lb == 1) KL(16, 1); else if (lb == 4) KL(16, 4); else KL(16, 2);
     } else {
         if (lb == 1) KL(32, 1); else if (lb == 4) KL(32, 4); else KL(32, 2);
     }
 #undef KL
+#undef KP
 }
 
 template <typename real_t>
@@ -796,7 +1062,7 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     // work-groups are dealt round-robin to the 8 XCDs: give each XCD a contiguous range of
     // (block, column slice) items so that neighbour blocks meet in the same L2
     int const v = sizeof(real_t) == 8 ? 0 : 1;
-    int const JW = d.tune_jw[v], LB = d.tune_lb[v];
+    int const KIND = d.tune_kind[v], JW = d.tune_jw[v], LB = d.tune_lb[v];
     if (a.nitems == 0) return 0;
     uint64_t const items = uint64_t(64 / JW) * a.nitems;
     uint64_t const per = (items + 7) / 8;
@@ -805,11 +1071,11 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     if (grid > 0x7fffffffull) { err = "too many blocks for one launch"; return 1; }
     dim3 const g{uint32_t(grid), 1, 1};
     if (d.nh_max == 2) {
-        if (d.has_images) launch_local<real_t, 2, true>(JW, LB, g, stream, a);
-        else launch_local<real_t, 2, false>(JW, LB, g, stream, a);
+        if (d.has_images) launch_local<real_t, 2, true>(KIND, JW, LB, g, stream, a);
+        else launch_local<real_t, 2, false>(KIND, JW, LB, g, stream, a);
     } else {
-        if (d.has_images) launch_local<real_t, 1, true>(JW, LB, g, stream, a);
-        else launch_local<real_t, 1, false>(JW, LB, g, stream, a);
+        if (d.has_images) launch_local<real_t, 1, true>(KIND, JW, LB, g, stream, a);
+        else launch_local<real_t, 1, false>(KIND, JW, LB, g, stream, a);
     }
     HIP_TRY(hipGetLastError());
     if (d.n_wide_pairs > 0) {
@@ -829,15 +1095,16 @@ template <typename real_t>
 int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream, int reps,
                     double* best_ms, std::string& err) {
     int const v = sizeof(real_t) == 8 ? 0 : 1;
-    static int const JWS[2] = {16, 32}, LBS[3] = {1, 2, 4};
+    static int const SHAPES[8][3] = {{0, 16, 1}, {0, 16, 2}, {0, 16, 4}, {0, 32, 1}, {0, 32, 2}, {0, 32, 4},
+                                     {1, 32, 0}, {1, 64, 0}};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     double best = 1e300;
-    int bjw = d.tune_jw[v], blb = d.tune_lb[v];
-    for (int jw : JWS)
-        for (int lb : LBS) {
-            d.tune_jw[v] = jw; d.tune_lb[v] = lb;
+    int bkind = d.tune_kind[v], bjw = d.tune_jw[v], blb = d.tune_lb[v];
+    for (auto const& sh : SHAPES) {
+            int const kind = sh[0], jw = sh[1], lb = sh[2];
+            d.tune_kind[v] = kind; d.tune_jw[v] = jw; d.tune_lb[v] = lb;
             if (int r = device_action<real_t>(d, s, y, x, stream, err)) return r;  // warm-up
             HIP_TRY(hipEventRecord(e0, stream));
             for (int k = 0; k < reps; ++k)
@@ -847,11 +1114,11 @@ int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t con
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
             double const t = double(ms) / std::max(1, reps);
-            if (t < best) { best = t; bjw = jw; blb = lb; }
-        }
+            if (t < best) { best = t; bkind = kind; bjw = jw; blb = lb; }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    d.tune_jw[v] = bjw; d.tune_lb[v] = blb;
+    d.tune_kind[v] = bkind; d.tune_jw[v] = bjw; d.tune_lb[v] = blb;
     d.tuned[v] = true;
     if (best_ms) *best_ms = best;
     return 0;

@@ -302,13 +302,17 @@ int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* 
     });
 }
 
-int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int columns, int lines) {
+int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int lines) {
     return guarded([&]() -> int {
         if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
         if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
-        if ((columns != 16 && columns != 32) || (lines != 1 && lines != 2 && lines != 4))
-            return fail(GFA_ERR_ARGUMENT, "tile shapes: 16 or 32 columns, 1, 2 or 4 lines");
+        if (kind == 0 && ((columns != 16 && columns != 32) || (lines != 1 && lines != 2 && lines != 4)))
+            return fail(GFA_ERR_ARGUMENT, "tile shapes of kind 0: 16 or 32 columns, 1, 2 or 4 lines");
+        if (kind == 1 && columns != 32 && columns != 64)
+            return fail(GFA_ERR_ARGUMENT, "tile shapes of kind 1: 32 or 64 columns");
+        if (kind != 0 && kind != 1) return fail(GFA_ERR_ARGUMENT, "tile shape kind must be 0 or 1");
         int const v = fp_bits == 64 ? 0 : 1;
+        plan->dev.tune_kind[v] = kind;
         plan->dev.tune_jw[v] = columns;
         plan->dev.tune_lb[v] = lines;
         plan->dev.tuned[v] = true;

@@ -36,7 +36,7 @@ struct DevicePlan {
     bool has_images = false;           // some finite-difference halo is a periodic image
     int fd_range[3] = {4, 4, 4};
     // k_local tile shape per precision (index 0: FP64, 1: FP32), set by the autotuner
-    int tune_jw[2] = {32, 32}, tune_lb[2] = {2, 2};
+    int tune_kind[2] = {0, 0}, tune_jw[2] = {32, 32}, tune_lb[2] = {2, 2};
     bool tuned[2] = {false, false};
 
     // index data

@@ -90,7 +90,7 @@ def lib():
         L.gfa_action_multiply.argtypes = [vp, C.c_int, vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int,
                                           C.POINTER(C.c_double)]
         L.gfa_autotune.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
-        L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.gfa_action_part.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gfa_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_double), C.POINTER(SolveStats)]
         L.gfa_last_error.restype = C.c_char_p
@@ -266,8 +266,8 @@ class GreenPlan:
         st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         _check(lib().gfa_action_part(self.h, fp_bits, part, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr()), st))
 
-    def set_tile_shape(self, fp_bits, columns, lines):
-        _check(lib().gfa_set_tile_shape(self.h, fp_bits, columns, lines))
+    def set_tile_shape(self, fp_bits, kind, columns, lines=0):
+        _check(lib().gfa_set_tile_shape(self.h, fp_bits, kind, columns, lines))
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
               shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0):

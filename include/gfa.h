@@ -148,8 +148,9 @@ int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* 
  *      layout of gfa_action; y is overwritten), reps applications each, and keeps the fastest for fp_bits.
  *      No counterpart in the reference, whose kernels have fixed launch shapes. ---- */
 int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, int reps, double* best_ms);
-/* pins the tile shape instead: columns per work-group 16 or 32, kinetic lines per thread in flight 1, 2 or 4 */
-int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int columns, int lines);
+/* pins the tile shape instead: kind 0 (LDS accumulator): columns per work-group 16 or 32, kinetic lines per
+ * thread in flight 1, 2 or 4; kind 1 (register accumulator per z plane): columns 32 or 64, lines ignored */
+int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int lines);
 
 /* ---- the solve (H - E S) X = B with B = unit columns on the diagonal blocks; replaces
  *      green_solver_t::solve / action_t::solve (include/green_solver.hxx:24-30, include/green_action.hxx:181-241)

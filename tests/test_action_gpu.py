@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from angstromcube_amd import gfa as G
+from angstromcube_amd import synthetic as S
 from oracle import oracle as O
 from tests.cases import cases
 
@@ -86,21 +87,25 @@ def test_action_rejects_bad_arguments(device):
         G._check(G.lib().gfa_action(plan.h, 64, G.C.c_void_p(xd.data_ptr()), G.C.c_void_p(xd.data_ptr()), None, None))
 
 
-@pytest.mark.parametrize("columns,lines", [(16, 1), (16, 2), (16, 4), (32, 1), (32, 2), (32, 4)])
-def test_every_tile_shape_matches_oracle(device, columns, lines):
+@pytest.mark.parametrize("kind,columns,lines", [(0, 16, 1), (0, 16, 2), (0, 16, 4), (0, 32, 1), (0, 32, 2), (0, 32, 4),
+                                               (1, 32, 0), (1, 64, 0)])
+def test_every_tile_shape_matches_oracle(device, kind, columns, lines):
     """Each tile shape the autotuner may pick gives the oracle's action (FP64 1e-12, FP32 2e-5), with periodic
     images (8th order) and with the 16th-order stencil."""
     import torch
-    for name, mk, kw, E, k in [c for c in cases(small_only=True) if c[0] in ("periodic8_k", "isolated6_fd16")]:
+    geoms = [c for c in cases() if c[0] in ("periodic8_k", "isolated6_fd16", "mixed_bc", "wrap16_numax5")]
+    geoms.append(("wrap8_fd8", lambda: S.make_system(9, (8, 8, 8), natoms=4, numax=3, bc=(1, 1, 1)),
+                  dict(r_trunc=3.0, nsources=(2, 1, 1), fd_range=(4, 4, 4)), complex(-0.4, 0.05), (0, 0, 0)))
+    for name, mk, kw, E, k in geoms:
         plan = G.GreenPlan(mk(), **kw)
         plan.set_inputs(E, k)
         x = _rand((plan.nnzb, 2, 64, 64), 21)
         y_ref = O.Oracle(plan, E, k).action(x)
         for fp, tol, dt in ((64, 1e-12, torch.float64), (32, 2e-5, torch.float32)):
-            plan.set_tile_shape(fp, columns, lines)
+            plan.set_tile_shape(fp, kind, columns, lines)
             xd = torch.from_numpy(x).to(device=device, dtype=dt)
             yd = torch.empty_like(xd)
-            print(f"tile {columns}x{lines} {name} fp{fp}", flush=True)
+            print(f"tile {kind}:{columns}x{lines} {name} fp{fp}", flush=True)
             plan.multiply(yd, xd, fp_bits=fp)
             torch.cuda.synchronize()
             err = np.abs(yd.double().cpu().numpy() - y_ref).max() / np.abs(y_ref).max()
@@ -122,7 +127,9 @@ def test_autotune_keeps_the_action_exact(device):
     y_ref = O.Oracle(plan, E, k).action(x)
     assert np.abs(yd.cpu().numpy() - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
     with pytest.raises(G.GfaError, match="tile"):
-        plan.set_tile_shape(64, 24, 2)
+        plan.set_tile_shape(64, 0, 24, 2)
+    with pytest.raises(G.GfaError, match="tile"):
+        plan.set_tile_shape(64, 1, 16)
 
 
 def test_action_parts_compose(device):
