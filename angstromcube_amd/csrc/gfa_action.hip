@@ -978,6 +978,109 @@ int device_upload_values(Plan const& p, DevicePlan& d, std::string& err) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// 4. adjoint gradient of L = Im sum_c w_c Tr[G_c], Tr over the diagonal block of column group c.
+//    With dG = -A^-1 dA G and lambda_c = A^-H P_c: dL = -Im sum_c w_c <lambda_c, dA G_c>.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// per block and point i: -Im(w_c sum_j conj(lambda[b][i][j]) G[b][i][j]), 0 outside the cell (no potential)
+__global__ __launch_bounds__(256) void k_grad_v_blocks(double* __restrict__ gpart, double const* __restrict__ X,
+                                                       double const* __restrict__ Lam, uint16_t const* __restrict__ colindx,
+                                                       int32_t const* __restrict__ veff, double const* __restrict__ wE) {
+    uint64_t const blk = blockIdx.x;
+    int const j = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t const c = colindx[blk];
+    double const wr = wE[4 * c], wi = wE[4 * c + 1];
+    bool const inside = veff[blk] >= 0;
+    for (int i = w; i < 64; i += 4) {
+        size_t const er = (blk * 2 + 0) * 4096 + i * 64 + j, ei = er + 4096;
+        double const gr = X[er], gi = X[ei], lr = Lam[er], li = Lam[ei];
+        double const sr = wave_sum(lr * gr + li * gi), si = wave_sum(lr * gi - li * gr);
+        if (j == 0) gpart[blk * 64 + i] = inside ? -(wr * si + wi * sr) : 0.0;
+    }
+}
+
+// per cell block: the sum over the blocks that read its potential, in ascending block order
+__global__ __launch_bounds__(64) void k_grad_v_cells(double* __restrict__ gcell, double const* __restrict__ gpart,
+                                                     uint32_t const* __restrict__ cell_start,
+                                                     uint32_t const* __restrict__ cell_blocks) {
+    uint32_t const v = blockIdx.x;
+    int const i = threadIdx.x;
+    double acc = 0;
+    for (uint32_t k = cell_start[v]; k < cell_start[v + 1]; ++k) acc += gpart[uint64_t(cell_blocks[k]) * 64 + i];
+    gcell[uint64_t(v) * 64 + i] = acc;
+}
+
+// atom coefficients of one vector: per (atom, column group) the image partials collected with the Bloch phase,
+// out[((atom_starts[a] + s) * nCols + c)][re/im][j] (the input of the atom-matrix multiply of k_sho_atoms)
+__global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_collect(
+    double* __restrict__ out, double const* __restrict__ partial, uint32_t const* __restrict__ chunk_coef,
+    uint32_t const* __restrict__ pair_chunk, uint32_t const* __restrict__ sum_start,
+    uint32_t const* __restrict__ sum_index, int8_t const* __restrict__ atom_lmax,
+    double const* __restrict__ image_phase, uint32_t nCols, uint32_t const* __restrict__ atom_starts) {
+    uint32_t const a = blockIdx.x, c = blockIdx.y;
+    int const j = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int const L = atom_lmax[a];
+    int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
+    for (int s = w; s < nc; s += ATOM_WAVES) {
+        double ar = 0, ai = 0;
+        for (uint32_t k = sum_start[a]; k < sum_start[a + 1]; ++k) {
+            uint32_t const img = sum_index[k];
+            double const ph_re = image_phase[2 * img], ph_im = image_phase[2 * img + 1];
+            uint32_t const c0 = pair_chunk[size_t(img) * nCols + c], c1 = pair_chunk[size_t(img) * nCols + c + 1];
+            double vr = 0, vi = 0;
+            for (uint32_t ch = c0; ch < c1; ++ch) {
+                double const* src = partial + (size_t(chunk_coef[ch]) + s) * 128 + j;
+                vr += src[0];
+                vi += src[64];
+            }
+            ar += ph_re * vr - ph_im * vi;
+            ai += ph_re * vi + ph_im * vr;
+        }
+        double* dst = out + ((size_t(atom_starts[a]) + s) * nCols + c) * 128 + j;
+        dst[0] = ar;
+        dst[64] = ai;
+    }
+}
+
+// per (atom a, row k): for every l, z_c = sum_j conj(cL[k][c][j]) cX[l][c][j];
+// gmat[a] = [-Im sum_c w_c z_c][k][l] (dL/dM entries times 1) and [-Im sum_c w_c (-E_c) z_c][k][l]
+__global__ __launch_bounds__(64) void k_grad_atoms(double* __restrict__ gmat, double const* __restrict__ cX,
+                                                   double const* __restrict__ cL, int8_t const* __restrict__ atom_lmax,
+                                                   uint32_t const* __restrict__ atom_starts,
+                                                   uint64_t const* __restrict__ atom_mat_off,
+                                                   double const* __restrict__ wE, uint32_t nCols) {
+    uint32_t const a = blockIdx.x;
+    int const k = blockIdx.y, j = threadIdx.x;
+    int const L = atom_lmax[a];
+    int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
+    if (k >= nc) return;
+    size_t const base = size_t(atom_starts[a]) * nCols * 128;
+    for (int l = 0; l < nc; ++l) {
+        double hr = 0, sr = 0;   // -Im of the two weighted sums
+        for (uint32_t c = 0; c < nCols; ++c) {
+            double const* pl = cL + base + (size_t(k) * nCols + c) * 128 + j;
+            double const* px = cX + base + (size_t(l) * nCols + c) * 128 + j;
+            double const lr = pl[0], li = pl[64], xr = px[0], xi = px[64];
+            double const zr = lr * xr + li * xi, zi = lr * xi - li * xr;   // conj(l) x
+            double const w_r = wE[4 * c], w_i = wE[4 * c + 1], v_r = wE[4 * c + 2], v_i = wE[4 * c + 3];
+            hr -= w_r * zi + w_i * zr;
+            sr -= v_r * zi + v_i * zr;
+        }
+        hr = wave_sum(hr);
+        sr = wave_sum(sr);
+        if (j == 0) {
+            gmat[atom_mat_off[a] + size_t(k) * nc + l] = hr;
+            gmat[atom_mat_off[a] + size_t(nc) * nc + size_t(k) * nc + l] = sr;
+        }
+    }
+}
+
 // Local-kernel tile shapes.  kind 0: k_local (LDS accumulator), JW columns per work-group (16 or 32), LB
 // kinetic lines per thread in flight (1, 2, 4).  kind 1: k_plane (register accumulator), JW = 32 or 64.
 template <typename real_t, int NH, bool IMG>
@@ -1125,5 +1228,80 @@ int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t con
 }
 template int device_autotune<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, int, double*, std::string&);
 template int device_autotune<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, int, double*, std::string&);
+
+// The adjoint gradient (FP64): X solves (H - E_c S) X = B, Lam solves (H - conj(E_c) S) Lam = B column group by
+// column group; wE[c] = {w_c, -E_c w_c}.  Results on the host: gcell_host[ncell][64] (dL/dV per cell point) and
+// gmat_host at atom_mat_off (per contributing atom [2][nc][nc], the derivative with respect to the entries of
+// (h dVol-scaled, s dVol-scaled) before the SHO prefactors, see gfa_gradient).
+int device_gradient(DevicePlan& d, Plan const& p, ActionScalars const& s, double const* X, double const* Lam,
+                    std::vector<double> const& wE, std::vector<double>& gcell_host, std::vector<double>& gmat_host,
+                    hipStream_t stream, std::string& err) {
+    auto& g = d.grad;
+    if (!g.ready) {
+        // cell -> blocks that read its potential, ascending block order
+        std::vector<uint32_t> start(p.ncell() + 1, 0), blocks;
+        for (uint64_t b = 0; b < p.nnzb; ++b) if (p.veff_index[b] >= 0) ++start[p.veff_index[b] + 1];
+        for (uint64_t v = 0; v < p.ncell(); ++v) start[v + 1] += start[v];
+        blocks.resize(start.back());
+        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+        for (uint64_t b = 0; b < p.nnzb; ++b) if (p.veff_index[b] >= 0) blocks[fill[p.veff_index[b]]++] = uint32_t(b);
+        int r;
+        if ((r = dupload(d, g.cell_start, start, err))) return r;
+        if ((r = dupload(d, g.cell_blocks, blocks, err))) return r;
+        if ((r = dalloc(d, g.gpart, size_t(p.nnzb) * 64, err))) return r;
+        if ((r = dalloc(d, g.gcell, size_t(p.ncell()) * 64, err))) return r;
+        if ((r = dalloc(d, g.wE, size_t(4) * p.nCols, err))) return r;
+        size_t const nco = std::max<size_t>(1, size_t(d.ncoeffs) * d.nCols * 128);
+        if ((r = dalloc(d, g.cX, nco, err))) return r;
+        if ((r = dalloc(d, g.cL, nco, err))) return r;
+        if ((r = dalloc(d, g.gmat, std::max<size_t>(1, p.atom_mat_off.back()), err))) return r;
+        g.ready = true;
+    }
+    HIP_TRY(hipMemcpyAsync(g.wE, wE.data(), wE.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_grad_v_blocks, dim3(uint32_t(p.nnzb)), dim3(256), 0, stream, g.gpart, X, Lam, d.colindx,
+                       d.veff, g.wE);
+    hipLaunchKernelGGL(k_grad_v_cells, dim3(uint32_t(p.ncell())), dim3(64), 0, stream, g.gcell, g.gpart, g.cell_start,
+                       g.cell_blocks);
+    HIP_TRY(hipGetLastError());
+    if (d.nAtoms > 0 && d.nchunks > 0) {
+        auto& ws = d.ws64;
+        if (!ws.partial) {
+            if (int r = dalloc(d, ws.partial, size_t(d.total_chunk_coefs) * 128, err)) return r;
+            if (int r = dalloc(d, ws.dcoef, size_t(d.natomcoeffs) * d.nCols * 128, err)) return r;
+        }
+        for (int v = 0; v < 2; ++v) {
+            double const* vec = v ? Lam : X;
+            for (int L = 0; L <= 7; ++L) {
+                uint32_t const n = d.bucket_start[L + 1] - d.bucket_start[L];
+                if (!n) continue;
+                uint32_t const* bucket = d.bucket_chunks + d.bucket_start[L];
+#define LAUNCH_PRJ(LL) hipLaunchKernelGGL((k_sho_project<double, LL>), dim3(n), dim3(128), 0, stream, ws.partial, vec, \
+                d.chunks, bucket, d.chunk_coef, d.prj_blocks, d.rowindx, d.row_pos, d.image_pos, \
+                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], (uint8_t const*)nullptr)
+                switch (L) {
+                    case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
+                    case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;
+                    case 4: LAUNCH_PRJ(4); break; case 5: LAUNCH_PRJ(5); break;
+                    case 6: LAUNCH_PRJ(6); break; default: LAUNCH_PRJ(7); break;
+                }
+#undef LAUNCH_PRJ
+            }
+            hipLaunchKernelGGL(k_sho_collect, dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), 0, stream, v ? g.cL : g.cX,
+                               ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.atom_lmax,
+                               d.image_phase, d.nCols, d.atom_starts);
+            HIP_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_grad_atoms, dim3(d.nAtoms, d.max_atom_nc), dim3(64), 0, stream, g.gmat, g.cX, g.cL,
+                           d.atom_lmax, d.atom_starts, d.atom_mat_off, g.wE, d.nCols);
+        HIP_TRY(hipGetLastError());
+    }
+    gcell_host.assign(size_t(p.ncell()) * 64, 0.0);
+    gmat_host.assign(p.atom_mat_off.back(), 0.0);
+    HIP_TRY(hipMemcpyAsync(gcell_host.data(), g.gcell, gcell_host.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (d.nAtoms > 0 && d.nchunks > 0 && !gmat_host.empty())
+        HIP_TRY(hipMemcpyAsync(gmat_host.data(), g.gmat, gmat_host.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+}
 
 }  // namespace gfa

@@ -302,6 +302,63 @@ int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* 
     });
 }
 
+int gfa_gradient(gfa_plan* plan, const void* X, const void* Lambda, const double* E, const double* w, double* grad_V,
+                 double* grad_hs, void* stream) {
+    return guarded([&]() -> int {
+        if (!plan || !X || !Lambda || !E || !w) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energies before the gradient");
+        if (int s = ensure_device(plan)) return s;
+        gfa::Plan const& h = plan->host;
+        std::vector<double> wE(size_t(4) * h.nCols);
+        for (uint32_t c = 0; c < h.nCols; ++c) {
+            double const wr = w[2 * c], wi = w[2 * c + 1], er = E[2 * c], ei = E[2 * c + 1];
+            wE[4 * c] = wr; wE[4 * c + 1] = wi;
+            wE[4 * c + 2] = -(er * wr - ei * wi); wE[4 * c + 3] = -(er * wi + ei * wr);   // -E w
+        }
+        std::vector<double> gcell, gmat;
+        std::string err;
+        auto const sc = scalars(h);
+        if (gfa::device_gradient(plan->dev, h, sc, (double const*)X, (double const*)Lambda, wE, gcell, gmat,
+                                 (hipStream_t)stream, err))
+            return fail(GFA_ERR_DEVICE, err);
+        if (grad_V) {   // cell-blocked -> grid [ng_z][ng_y][ng_x] (the inverse of gfa_set_potential's blocking)
+            size_t const ngx = 4 * size_t(h.nb[0]), ngy = 4 * size_t(h.nb[1]);
+            for (uint32_t bz = 0; bz < h.nb[2]; ++bz)
+            for (uint32_t by = 0; by < h.nb[1]; ++by)
+            for (uint32_t bx = 0; bx < h.nb[0]; ++bx) {
+                double const* src = &gcell[((size_t(bz) * h.nb[1] + by) * h.nb[0] + bx) * 64];
+                for (int z = 0; z < 4; ++z)
+                    for (int y = 0; y < 4; ++y)
+                        for (int x = 0; x < 4; ++x)
+                            grad_V[((bz * 4 + z) * ngy + (by * 4 + y)) * ngx + (bx * 4 + x)] = src[(z * 4 + y) * 4 + x];
+            }
+        }
+        if (grad_hs) {  // per original atom [2][nc][nc]: dL/dh and dL/ds of the matrices given to gfa_set_atom_matrices
+            size_t const natoms = h.xyzZinso.size() / 8;
+            std::vector<size_t> off(natoms + 1, 0);
+            for (size_t ia = 0; ia < natoms; ++ia) {
+                size_t const nc = gfa::nsho(int(h.xyzZinso[ia * 8 + 5]));
+                off[ia + 1] = off[ia] + 2 * nc * nc;
+            }
+            std::fill(grad_hs, grad_hs + off[natoms], 0.0);
+            for (uint32_t a = 0; a < h.nAtoms; ++a) {
+                int const ia = h.atom_orig[a];
+                auto const f = h.sho_prefactors(ia);
+                size_t const nc = f.size();
+                double const* gm = &gmat[h.atom_mat_off[a]];
+                double* out = grad_hs + off[ia];
+                for (size_t k = 0; k < nc; ++k)
+                    for (size_t l = 0; l < nc; ++l) {
+                        double const ff = f[k] * f[l] * h.dVol;
+                        out[k * nc + l] += h.scale_H * ff * gm[k * nc + l];
+                        out[nc * nc + k * nc + l] += ff * gm[nc * nc + k * nc + l];
+                    }
+            }
+        }
+        return GFA_OK;
+    });
+}
+
 int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int lines) {
     return guarded([&]() -> int {
         if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");

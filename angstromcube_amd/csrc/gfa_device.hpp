@@ -27,6 +27,18 @@ struct Workspace {
     size_t partial_elems = 0, dcoef_elems = 0;
 };
 
+struct GradWs {  // buffers of the adjoint gradient
+    bool ready = false;
+    uint32_t* cell_start = nullptr;   // [ncell+1] blocks reading each cell block's potential
+    uint32_t* cell_blocks = nullptr;
+    double* gpart = nullptr;          // [nnzb][64]
+    double* gcell = nullptr;          // [ncell][64]
+    double* wE = nullptr;             // [nCols][4]: w_c, -E_c w_c
+    double* cX = nullptr;             // [ncoeffs][nCols][2][64] collected atom coefficients of X
+    double* cL = nullptr;             // same for lambda
+    double* gmat = nullptr;           // per contributing atom [2][nc][nc]
+};
+
 struct DevicePlan {
     bool uploaded = false;
     int device = -1;
@@ -83,6 +95,7 @@ struct DevicePlan {
     double* image_phase = nullptr;     // [nai][2]
 
     Workspace<double> ws64;
+    GradWs grad;
     Workspace<float> ws32;
     size_t bytes = 0;
     std::vector<void*> allocations;
@@ -111,6 +124,9 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
 template <typename real_t>
 int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream, int reps,
                     double* best_ms, std::string& err);
+int device_gradient(DevicePlan& d, Plan const& p, ActionScalars const& s, double const* X, double const* Lam,
+                    std::vector<double> const& wE, std::vector<double>& gcell_host, std::vector<double>& gmat_host,
+                    hipStream_t stream, std::string& err);
 double reference_flops(Plan const& p, int fp_bits);
 
 }  // namespace gfa

@@ -547,15 +547,12 @@ int Plan::set_potential(double const* V, std::string& err) {
     return 0;
 }
 
-int Plan::set_atom_matrices(int ia, double const* hs, int nc, std::string& err) {
-    int const natoms = int(xyzZinso.size() / 8);
-    if (ia < 0 || ia >= natoms) { err = "atom index out of range"; return 1; }
+// prepare for unnormalized Hermite functions: M_ij * f_i * f_j, zyx order
+// (src/energy_contour.cxx:307-323 with include/sho_projection.hxx:181-209)
+std::vector<double> Plan::sho_prefactors(int ia) const {
     int const numax = int(xyzZinso[ia * 8 + 5]);
-    if (nc != nsho(numax)) { err = "matrix size does not match nSHO(numax) of this atom"; return 1; }
     double const sigma = xyzZinso[ia * 8 + 6];
-    // prepare for unnormalized Hermite functions: M_ij * f_i * f_j, zyx order
-    // (src/energy_contour.cxx:307-323 with include/sho_projection.hxx:181-209)
-    std::vector<double> f(nc);
+    std::vector<double> f(nsho(numax));
     int k = 0;
     for (int nz = 0; nz <= numax; ++nz) {
         double const fz = sho_1d_norm(nz, sigma);
@@ -564,6 +561,15 @@ int Plan::set_atom_matrices(int ia, double const* hs, int nc, std::string& err) 
             for (int nx = 0; nx <= numax - nz - ny; ++nx) f[k++] = sho_1d_norm(nx, sigma) * fy * fz;
         }
     }
+    return f;
+}
+
+int Plan::set_atom_matrices(int ia, double const* hs, int nc, std::string& err) {
+    int const natoms = int(xyzZinso.size() / 8);
+    if (ia < 0 || ia >= natoms) { err = "atom index out of range"; return 1; }
+    int const numax = int(xyzZinso[ia * 8 + 5]);
+    if (nc != nsho(numax)) { err = "matrix size does not match nSHO(numax) of this atom"; return 1; }
+    std::vector<double> const f = sho_prefactors(ia);
     auto& out = hs_scaled[ia];
     out.assign(2ull * nc * nc, 0.0);
     for (int m = 0; m < 2; ++m)

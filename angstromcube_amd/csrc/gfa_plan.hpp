@@ -120,6 +120,7 @@ public:
     int set_energies(std::complex<double> const* E, double dVol, std::string& err);  // one per column
     int set_kpoint(double const k[3], std::string& err);
     void update_atom_matrices();
+    std::vector<double> sho_prefactors(int ia) const;  // f_i of original atom ia, zyx order
 
 private:
     int build_bsr(PlanInput const& in, std::string& err,

@@ -24,7 +24,7 @@ EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS",
 EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
-             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_action_part",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_action_part", "gfa_gradient",
              "gfa_solve", "gfa_last_error"]
 
 
@@ -92,6 +92,8 @@ def lib():
         L.gfa_autotune.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
         L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.gfa_action_part.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+        L.gfa_gradient.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), vp]
         L.gfa_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_double), C.POINTER(SolveStats)]
         L.gfa_last_error.restype = C.c_char_p
         for name in C_SYMBOLS:
@@ -107,6 +109,11 @@ def _check(status):
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def nsho_of(numax):
+    n = int(numax)
+    return (n + 1) * (n + 2) * (n + 3) // 6
 
 
 class GreenPlan:
@@ -265,6 +272,30 @@ class GreenPlan:
         import torch
         st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         _check(lib().gfa_action_part(self.h, fp_bits, part, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr()), st))
+
+    def gradient(self, X, Lam, E, w, stream=None):
+        """dL/dV (grid array) and dL/d(h, s) per atom (list of (2, nc, nc)) of L = Im sum_c w_c Tr[G_c], from the
+        FP64 solutions X of the forward and Lam of the adjoint system (device tensors in the plan layout)."""
+        import torch
+        sysm = self.system
+        E = np.broadcast_to(np.asarray(E, dtype=complex), (self.nCols,))
+        w = np.broadcast_to(np.asarray(w, dtype=complex), (self.nCols,))
+        e = np.ascontiguousarray(np.stack([E.real, E.imag], axis=1))
+        ww = np.ascontiguousarray(np.stack([w.real, w.imag], axis=1))
+        gV = np.zeros(tuple(reversed(sysm.ng)))
+        ncs = [nsho_of(sysm.xyzZinso[ia, 5]) for ia in range(sysm.natoms)]
+        ghs = np.zeros(sum(2 * n * n for n in ncs))
+        st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+        for t in (X, Lam):
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (self.nnzb, 2, 64, 64):
+                raise GfaError("X and Lambda must be contiguous FP64 device tensors of shape (nnzb, 2, 64, 64)")
+        _check(lib().gfa_gradient(self.h, C.c_void_p(X.data_ptr()), C.c_void_p(Lam.data_ptr()), _dptr(e), _dptr(ww),
+                                  _dptr(gV), _dptr(ghs), st))
+        out, o = [], 0
+        for n in ncs:
+            out.append(ghs[o:o + 2 * n * n].reshape(2, n, n))
+            o += 2 * n * n
+        return gV, out
 
     def set_tile_shape(self, fp_bits, kind, columns, lines=0):
         _check(lib().gfa_set_tile_shape(self.h, fp_bits, kind, columns, lines))

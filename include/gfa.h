@@ -191,6 +191,18 @@ typedef struct {
  * may be NULL. */
 int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve_stats* stats);
 
+/* ---- adjoint gradient of L = Im sum_c w_c Tr[G_c], the trace over the diagonal block of column group c ----
+ *  With dG = -A^-1 dA G:  dL = -Im sum_c w_c <lambda_c, dA G_c>,  lambda_c = (H - E_c S)^-H B_c.
+ *  X: FP64 solution of (H - E_c S) X = B; Lambda: FP64 solution of the adjoint system (H - conj(E_c) S) Lambda = B
+ *  (the same plan with gfa_set_energies(conj(E))), both device arrays in the plan layout.  E: the forward
+ *  energies [nCols][2]; w: weights [nCols][2].  Outputs on the host (either may be NULL): grad_V[ng_z][ng_y][ng_x]
+ *  = dL/dV(r); grad_hs = per original atom [2][nc][nc] concatenated in atom order, dL/dh and dL/ds of the matrices
+ *  given to gfa_set_atom_matrices.  The V part is a fused conj(lambda) G reduction per block, folded per cell
+ *  block in a fixed order; the matrix part contracts the projected coefficients of lambda and X.
+ *  No counterpart in the reference (which has no derivatives). ---- */
+int gfa_gradient(gfa_plan* plan, const void* X, const void* Lambda, const double* E, const double* w, double* grad_V,
+                 double* grad_hs, void* stream);
+
 /* text of the last error of the calling thread ("" if none) */
 const char* gfa_last_error(void);
 

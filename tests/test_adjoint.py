@@ -1,0 +1,203 @@
+"""Adjoint gradient of L = Im sum_c w_c Tr[G_c] with respect to the grid potential and the atom matrices.
+
+Pins: (1) on the CPU, the adjoint formula dL/dV = -Im sum_c w_c conj(lambda_c) G_c (lambda_c = A_c^-H B_c) evaluated
+with dense operators assembled from the oracle's action equals central differences of the dense forward solve;
+(2) on the GPU, gfa_gradient with the device solves equals that CPU FP64 adjoint for dL/dV and for dL/dh, dL/ds
+(the latter from oracle action differences, exact since A is affine in h and s); (3) the torch.autograd wrapper
+equals central differences of GPU forward solves (step 1e-6 Ha) to 1e-6 relative to the largest gradient component
+(the difference quotient itself carries the solve error of ~1e-12 |L| divided by the step, so a component-wise
+1e-6 bound is out of reach of FP64 solves for the small components)."""
+import copy
+
+import numpy as np
+import pytest
+
+from angstromcube_amd import gfa as G
+from angstromcube_amd import synthetic as S
+from oracle import oracle as O
+
+E2 = np.array([-0.5 + 0.2j, -0.3 + 0.35j])
+W2 = np.array([0.7 - 0.2j, 0.4 + 0.5j])
+
+
+def small_system():
+    return S.make_system(0xAD10, (4, 4, 4), ngauss=4, natoms=2, numax=2)
+
+
+def small_plan(sysm):
+    # two instances of the same source block, untruncated periodic cell (Gamma)
+    return G.GreenPlan(sysm, r_trunc=6.0, sources=[(2, 2, 2), (2, 2, 2)], source_radii=[6.0, 6.0])
+
+
+def group_index(plan):
+    cols, sub = plan.export("COLINDX"), plan.export("SUBSET")
+    return [np.nonzero(cols == c)[0] for c in range(plan.nCols)], sub
+
+
+def dense_operator(plan, sysm, E, c, blocks):
+    """A_c as a dense complex matrix over (block of group c, point), built from oracle actions on unit vectors."""
+    ora = O.Oracle(plan, E)
+    n = len(blocks) * 64
+    A = np.zeros((n, n), dtype=complex)
+    for start in range(0, n, 64):
+        x = np.zeros((plan.nnzb, 2, 64, 64))
+        for j in range(64):
+            u = start + j
+            x[blocks[u // 64], 0, u % 64, j] = 1.0
+        y = ora.action(x)
+        for j in range(64):
+            A[:, start + j] = (y[blocks, 0, :, j] + 1j * y[blocks, 1, :, j]).reshape(-1)
+    return A
+
+
+def dense_solutions(plan, sysm, E):
+    """G_c and lambda_c = A_c^-H B_c for every column group, dense, and the operators."""
+    groups, sub = group_index(plan)
+    out = []
+    for c, blocks in enumerate(groups):
+        A = dense_operator(plan, sysm, E[c], c, blocks)
+        d = int(np.nonzero(blocks == sub[c])[0][0])
+        B = np.zeros((A.shape[0], 64))
+        B[d * 64 + np.arange(64), np.arange(64)] = 1.0
+        out.append((A, B, np.linalg.solve(A, B), np.linalg.solve(A.conj().T, B), blocks, d))
+    return out
+
+
+def trace_L(sols, w):
+    return sum(float(np.imag(w[c] * np.trace(Gc[d * 64:(d + 1) * 64]))) for c, (A, B, Gc, Lc, blocks, d) in enumerate(sols))
+
+
+def dense_grad_V(plan, sols, w):
+    """-Im sum_c w_c sum_j conj(lambda) G per (block, point), folded onto cell-blocked potential points."""
+    veff = plan.export("VEFF_INDEX")
+    ncell = int(np.prod([n // 4 for n in plan.system.ng]))
+    g = np.zeros((ncell, 64))
+    for c, (A, B, Gc, Lc, blocks, d) in enumerate(sols):
+        s = (np.conj(Lc) * Gc).sum(axis=1).reshape(len(blocks), 64)
+        for k, b in enumerate(blocks):
+            if veff[b] >= 0:
+                g[veff[b]] += -np.imag(w[c] * s[k])
+    return g
+
+
+def cell_to_grid(g, nb):
+    V = np.zeros((4 * nb[2], 4 * nb[1], 4 * nb[0]))
+    for bz in range(nb[2]):
+        for by in range(nb[1]):
+            for bx in range(nb[0]):
+                V[4 * bz:4 * bz + 4, 4 * by:4 * by + 4, 4 * bx:4 * bx + 4] = \
+                    g[(bz * nb[1] + by) * nb[0] + bx].reshape(4, 4, 4)
+    return V
+
+
+def test_dense_adjoint_formula_equals_central_differences():
+    """CPU only: the adjoint expression for dL/dV against central differences of dense forward solves."""
+    sysm = small_system()
+    plan = small_plan(sysm)
+    sols = dense_solutions(plan, sysm, E2)
+    g = dense_grad_V(plan, sols, W2)
+    veff = plan.export("VEFF_INDEX")
+    rng = np.random.default_rng(5)
+    delta = 1e-5
+    for _ in range(4):
+        cell, i = int(rng.integers(g.shape[0])), int(rng.integers(64))
+        Ls = []
+        for sgn in (+1, -1):
+            pert = []
+            for c, (A, B, Gc, Lc, blocks, d) in enumerate(sols):
+                Ap = A.copy()
+                for k, b in enumerate(blocks):
+                    if veff[b] == cell:
+                        Ap[k * 64 + i, k * 64 + i] += sgn * delta
+                Gp = np.linalg.solve(Ap, B)
+                pert.append((Ap, B, Gp, None, blocks, d))
+            Ls.append(trace_L(pert, W2))
+        fd = (Ls[0] - Ls[1]) / (2 * delta)
+        assert abs(fd - g[cell, i]) <= 1e-6 * np.abs(g).max(), (cell, i, fd, g[cell, i])
+
+
+class _PlanView:
+    """The plan's index structures with another system's values (the oracle reads both)."""
+
+    def __init__(self, plan, system):
+        self.system, self.sizes, self.export = system, plan.sizes, plan.export
+
+
+def oracle_grad_matrix(plan, sysm, sols, E, w, ia, which, k, l):
+    """-Im sum_c w_c <lambda_c, (dA/dM) G_c> for entry (k, l) of h (which 0) or s (which 1) of atom ia, with the
+    operator difference A(M + e_kl) - A(M) from oracle actions (exact: A is affine in h and s)."""
+    groups, _ = group_index(plan)
+    total = 0.0
+    for c, (A, B, Gc, Lc, blocks, d) in enumerate(sols):
+        Gx = np.zeros((plan.nnzb, 2, 64, 64))
+        Gx[blocks, 0] = Gc.real.reshape(len(blocks), 64, 64)
+        Gx[blocks, 1] = Gc.imag.reshape(len(blocks), 64, 64)
+        s_plus = copy.deepcopy(sysm)
+        s_plus.hs[ia] = s_plus.hs[ia].copy()
+        s_plus.hs[ia][which, k, l] += 1.0
+        y1 = O.Oracle(_PlanView(plan, s_plus), E[c]).action(Gx)
+        y0 = O.Oracle(plan, E[c]).action(Gx)
+        dAG = (y1[blocks, 0] - y0[blocks, 0]) + 1j * (y1[blocks, 1] - y0[blocks, 1])
+        total += -np.imag(w[c] * np.sum(np.conj(Lc).reshape(len(blocks), 64, 64) * dAG))
+    return total
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_equals_cpu_adjoint(device):
+    import torch
+    sysm = small_system()
+    plan = small_plan(sysm)
+    plan.update_potential()
+    from angstromcube_amd import adjoint as A
+    X, _ = A.solve_fp64(plan, E2, 1e-12, 3000)
+    Lam, _ = A.solve_fp64(plan, np.conj(E2), 1e-12, 3000)
+    plan.update_energies(E2)
+    gV, ghs = plan.gradient(X, Lam, E2, W2)
+    torch.cuda.synchronize()
+    sols = dense_solutions(plan, sysm, E2)
+    gV_ref = cell_to_grid(dense_grad_V(plan, sols, W2), [n // 4 for n in sysm.ng])
+    err = np.abs(gV - gV_ref).max() / np.abs(gV_ref).max()
+    print(f"dL/dV: GPU vs dense CPU adjoint, max rel {err:.2e}")
+    assert err < 1e-7
+    for ia, which, k, l in [(0, 0, 0, 0), (0, 0, 1, 3), (1, 1, 2, 2), (1, 1, 0, 5), (0, 1, 4, 1)]:
+        ref = oracle_grad_matrix(plan, sysm, sols, E2, W2, ia, which, k, l)
+        got = ghs[ia][which, k, l]
+        print(f"dL/d{'hs'[which]}[{ia}][{k},{l}]: GPU {got:.10e} CPU {ref:.10e}")
+        assert abs(got - ref) <= 1e-7 * max(abs(ref), 1e-3 * np.abs(ghs[ia]).max())
+
+
+@pytest.mark.gpu
+def test_autograd_equals_central_differences(device):
+    import torch
+    from angstromcube_amd import adjoint as A
+    sysm = small_system()
+    plan = small_plan(sysm)
+    V = torch.from_numpy(sysm.V.copy()).requires_grad_(True)
+    hs = torch.from_numpy(A.flatten_hs(sysm.hs)).requires_grad_(True)
+    L = A.contour_trace(V, hs, plan, E2, W2, tol=1e-12)
+    gV, ghs = torch.autograd.grad(L, (V, hs))
+    rng = np.random.default_rng(7)
+    step = 1e-6
+    pts = [tuple(int(rng.integers(n)) for n in V.shape) for _ in range(8)]
+    ents = [int(rng.integers(hs.numel())) for _ in range(4)]
+    with torch.no_grad():
+        for p in pts:
+            Lp, Lm = [float(A.contour_trace(V + sgn * step * _unit(V, p), hs, plan, E2, W2, tol=1e-12))
+                      for sgn in (+1, -1)]
+            fd = (Lp - Lm) / (2 * step)
+            print(f"V{p}: adjoint {float(gV[p]):.8e} fd {fd:.8e}")
+            assert abs(fd - float(gV[p])) <= 1e-6 * float(gV.abs().max())
+        for e in ents:
+            u = torch.zeros_like(hs)
+            u[e] = 1.0
+            Lp, Lm = [float(A.contour_trace(V, hs + sgn * step * u, plan, E2, W2, tol=1e-12)) for sgn in (+1, -1)]
+            fd = (Lp - Lm) / (2 * step)
+            print(f"hs[{e}]: adjoint {float(ghs[e]):.8e} fd {fd:.8e}")
+            assert abs(fd - float(ghs[e])) <= 1e-6 * float(ghs.abs().max())
+
+
+def _unit(V, p):
+    import torch
+    u = torch.zeros_like(V)
+    u[p] = 1.0
+    return u
