@@ -384,13 +384,12 @@ int Plan::build_dyadic(std::string& err) {
     for (int32_t ia = 0; ia < natoms; ++ia) max_sigma = std::max(max_sigma, xyzZinso[ia * 8 + 6]);
     double const radius = r_trunc + max_distance_from_center + 2 * max_sigma * r_proj + 2 * r_block_circ;
     int images[3] = {0, 0, 0}, copies[3] = {0, 0, 0};
-    uint32_t nimages = 1, ncopies = 1;
+    uint32_t ncopies = 1;
     for (int d = 0; d < 3; ++d) {
         int const nmx = int(std::max(0., std::ceil(radius / cell[d])));
         if (BC_PERIODIC == bc[d]) images[d] = nmx;
         else if (BC_REPEAT == bc[d]) copies[d] = nmx;
         else if (BC_WRAP == bc[d]) copies[d] = 1;
-        nimages *= uint32_t(2 * images[d] + 1);
         ncopies *= uint32_t(2 * copies[d] + 1);
     }
     size_t const ncopies_all = size_t(natoms) * ncopies;

@@ -405,6 +405,130 @@ __global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, 
 }
 
 // ------------------------------------------------------------------------------------------------
+// COCG (conjugate orthogonal CG for complex-symmetric A, van der Vorst & Melissen, IEEE Trans. Magn. 26, 706
+// (1990)): CG with the bilinear form (a, b) = sum a_i b_i.  Vectors r, p, q = A p; the solution in FP64.
+// ------------------------------------------------------------------------------------------------
+// r = p = b; partials |b|^2, Re (b, b), Im (b, b)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cocg_init(T* __restrict__ rv, T* __restrict__ p, T const* __restrict__ rhs,
+                                                   uint8_t const* __restrict__ diag, SegArgs g) {
+    SEG_PROLOGUE
+    double acc[3][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+        bool const dg = diag[blk];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T br[V], bi[V];
+            if (rhs) { ld(br, rhs + OFF(0, i)); ld(bi, rhs + OFF(1, i)); }
+            else {
+#pragma unroll
+                for (int t = 0; t < V; ++t) { br[t] = (dg && i == j0 + t) ? T(1) : T(0); bi[t] = 0; }
+            }
+            st(rv + OFF(0, i), br); st(rv + OFF(1, i), bi);
+            st(p + OFF(0, i), br); st(p + OFF(1, i), bi);
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                double const xr = br[t], xi = bi[t];
+                acc[0][t] += xr * xr + xi * xi;
+                acc[1][t] += xr * xr - xi * xi;
+                acc[2][t] += 2 * xr * xi;
+            }
+        }
+    }
+    store_partials<T, 3>(acc, g.segpart, s);
+}
+
+// partials Re (p, q), Im (p, q)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cocg_pq(T const* __restrict__ p, T const* __restrict__ q, SegArgs g) {
+    SEG_PROLOGUE
+    double acc[2][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T pr[V], pi[V], qr[V], qi[V];
+            ld(pr, p + OFF(0, i)); ld(pi, p + OFF(1, i));
+            ld(qr, q + OFF(0, i)); ld(qi, q + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                acc[0][t] += double(pr[t]) * qr[t] - double(pi[t]) * qi[t];
+                acc[1][t] += double(pr[t]) * qi[t] + double(pi[t]) * qr[t];
+            }
+        }
+    }
+    store_partials<T, 2>(acc, g.segpart, s);
+}
+
+// x += alpha p (FP64), r -= alpha q; partials |r|^2, Re (r, r), Im (r, r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cocg_xr(double* __restrict__ x, T* __restrict__ rv, T const* __restrict__ p,
+                                                 T const* __restrict__ q, double const* __restrict__ S, uint32_t ncol,
+                                                 SegArgs g) {
+    SEG_PROLOGUE
+    Cx<double> a[V];
+    coefs(a, S, ncol, q0, ALPHA_RE);
+    double acc[3][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T pr[V], pi[V], qr[V], qi[V], rr[V], ri[V];
+            double xr[V], xi[V];
+            ld(pr, p + OFF(0, i)); ld(pi, p + OFF(1, i));
+            ld(qr, q + OFF(0, i)); ld(qi, q + OFF(1, i));
+            ld(rr, rv + OFF(0, i)); ld(ri, rv + OFF(1, i));
+            ldd<V>(xr, x + OFF(0, i)); ldd<V>(xi, x + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                xr[t] += a[t].re * double(pr[t]) - a[t].im * double(pi[t]);
+                xi[t] += a[t].re * double(pi[t]) + a[t].im * double(pr[t]);
+                T const ar = T(a[t].re), ai = T(a[t].im);
+                rr[t] = rr[t] - (ar * qr[t] - ai * qi[t]);
+                ri[t] = ri[t] - (ar * qi[t] + ai * qr[t]);
+                double const vr = rr[t], vi = ri[t];
+                acc[0][t] += vr * vr + vi * vi;
+                acc[1][t] += vr * vr - vi * vi;
+                acc[2][t] += 2 * vr * vi;
+            }
+            std_<V>(x + OFF(0, i), xr); std_<V>(x + OFF(1, i), xi);
+            st(rv + OFF(0, i), rr); st(rv + OFF(1, i), ri);
+        }
+    }
+    store_partials<T, 3>(acc, g.segpart, s);
+}
+
+// p = r + beta p
+template <typename T>
+__global__ __launch_bounds__(256) void k_cocg_p(T* __restrict__ p, T const* __restrict__ rv, double const* __restrict__ S,
+                                                uint32_t ncol, SegArgs g) {
+    SEG_PROLOGUE
+    Cx<T> be[V];
+    coefs(be, S, ncol, q0, BETA_RE);
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T pr[V], pi[V], rr[V], ri[V];
+            ld(pr, p + OFF(0, i)); ld(pi, p + OFF(1, i));
+            ld(rr, rv + OFF(0, i)); ld(ri, rv + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                T const nr = rr[t] + (be[t].re * pr[t] - be[t].im * pi[t]);
+                T const ni = ri[t] + (be[t].re * pi[t] + be[t].im * pr[t]);
+                pr[t] = nr; pi[t] = ni;
+            }
+            st(p + OFF(0, i), pr); st(p + OFF(1, i), pi);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // per-column scalar updates; partials are folded over the column group's segments in a fixed order
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ double& F(double* S, uint32_t ncol, int f, uint32_t q) { return S[uint64_t(f) * ncol + q]; }
@@ -560,6 +684,81 @@ __global__ __launch_bounds__(64 * FOLD_P) void k_scalar_theta(double* __restrict
         F(S, ncol, FE, q) = 0; F(S, ncol, FE + 1, q) = 0;
         F(S, ncol, FC, q) = 0; F(S, ncol, FC + 1, q) = 0;
     }
+}
+
+// COCG: after k_cocg_init: |b|^2, (b, b); tolerance and status as k_scalar_init
+__global__ __launch_bounds__(64 * FOLD_P) void k_cocg_scalar_init(double* __restrict__ S, double const* __restrict__ segpart,
+                                                               uint32_t const* __restrict__ col_seg,
+                                                               uint8_t const* __restrict__ col_active, uint32_t ncol,
+                                                               double tol_rel, double const* __restrict__ tol_abs) {
+    uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
+    if (!col_active[c]) {
+        if (threadIdx.x < 64) {
+            F(S, ncol, STATUS, q) = CONVERGED;
+            F(S, ncol, MSTEP, q) = 0;
+            freeze_step(S, ncol, q);
+        }
+        return;
+    }
+    int const f[3] = {0, 1, 2};
+    double t[3];
+    fold_group<3>(segpart, col_seg, c, f, t);
+    if (threadIdx.x >= 64) return;
+    double const bn = sqrt(t[0]);
+    F(S, ncol, BNORM, q) = bn;
+    F(S, ncol, RHO_RE, q) = t[1];
+    F(S, ncol, RHO_IM, q) = t[2];
+    double const tol = fmax(tol_rel * bn, tol_abs ? tol_abs[q] : 0.0);
+    F(S, ncol, TOL, q) = tol;
+    F(S, ncol, MSTEP, q) = 0;
+    freeze_step(S, ncol, q);
+    F(S, ncol, STATUS, q) = (bn <= tol) ? CONVERGED : ACTIVE;
+}
+
+// COCG: alpha = (r, r) / (p, A p)
+__global__ __launch_bounds__(64 * FOLD_P) void k_cocg_alpha(double* __restrict__ S, double const* __restrict__ segpart,
+                                                         uint32_t const* __restrict__ col_seg,
+                                                         uint8_t const* __restrict__ col_active, uint32_t ncol) {
+    uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
+    if (!col_active[c]) return;
+    int const f[2] = {0, 1};
+    double t[2];
+    fold_group<2>(segpart, col_seg, c, f, t);
+    if (threadIdx.x >= 64) return;
+    if (F(S, ncol, STATUS, q) != ACTIVE) { freeze_step(S, ncol, q); return; }
+    double const mr = t[0], mi = t[1], den = mr * mr + mi * mi;
+    if (!(den > 0)) { F(S, ncol, STATUS, q) = BREAKDOWN; freeze_step(S, ncol, q); return; }
+    double const rr = F(S, ncol, RHO_RE, q), ri = F(S, ncol, RHO_IM, q);
+    F(S, ncol, ALPHA_RE, q) = (rr * mr + ri * mi) / den;
+    F(S, ncol, ALPHA_IM, q) = (ri * mr - rr * mi) / den;
+}
+
+// COCG: after the r update: convergence on the recursive residual |r| <= tol; beta = (r', r') / (r, r)
+__global__ __launch_bounds__(64 * FOLD_P) void k_cocg_beta(double* __restrict__ S, double const* __restrict__ segpart,
+                                                        uint32_t const* __restrict__ col_seg,
+                                                        uint8_t const* __restrict__ col_active, uint32_t ncol) {
+    uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
+    if (!col_active[c]) return;
+    int const f[3] = {0, 1, 2};
+    double t[3];
+    fold_group<3>(segpart, col_seg, c, f, t);
+    if (threadIdx.x >= 64) return;
+    if (F(S, ncol, STATUS, q) != ACTIVE) { F(S, ncol, BETA_RE, q) = 0; F(S, ncol, BETA_IM, q) = 0; return; }
+    F(S, ncol, MSTEP, q) = F(S, ncol, MSTEP, q) + 1;
+    double const rn = sqrt(t[0]);
+    double const rr = F(S, ncol, RHO_RE, q), ri = F(S, ncol, RHO_IM, q), den = rr * rr + ri * ri;
+    if (rn <= F(S, ncol, TOL, q)) {
+        F(S, ncol, STATUS, q) = CONVERGED;   // x already holds this step
+        F(S, ncol, BETA_RE, q) = 0; F(S, ncol, BETA_IM, q) = 0;
+    } else if (!(den > 0) || !(rn == rn)) {
+        F(S, ncol, STATUS, q) = BREAKDOWN;
+        F(S, ncol, BETA_RE, q) = 0; F(S, ncol, BETA_IM, q) = 0;
+    } else {
+        F(S, ncol, BETA_RE, q) = (t[1] * rr + t[2] * ri) / den;
+        F(S, ncol, BETA_IM, q) = (t[2] * rr - t[1] * ri) / den;
+    }
+    F(S, ncol, RHO_RE, q) = t[1];
+    F(S, ncol, RHO_IM, q) = t[2];
 }
 
 // residual norms of the active column groups (others keep their value)
@@ -782,6 +981,65 @@ int enqueue_step(gfa_plan* plan, SolverState& st, Vecs<T> const& V, uint64_t see
     return 0;
 }
 
+// Runs steps (each `halfsteps_per_step` half-steps = actions) until every column stopped or maxit half-steps:
+// replays a captured HIP graph of one step (eager: plain launches), polls the column status every
+// `steps_per_check` steps, retires finished column groups and re-captures for the smaller active set.
+template <typename StepFn>
+int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int steps_per_check, int halfsteps_per_step,
+            int fp, uint64_t key, bool eager, StepFn&& step, TfqmrResult& res, std::string& err) {
+    hipStream_t const s = st.stream;
+    int m = 0;
+    while (m < maxit) {
+        int const nsteps = std::min(steps_per_check, (maxit - m + halfsteps_per_step - 1) / halfsteps_per_step);
+        if (eager) {
+            for (int k = 0; k < nsteps; ++k)
+                if (int r = step(s)) return r;
+        } else {
+            if (!st.graph || st.graph_fp != fp || st.graph_seed != key) {
+                drop_graph(st);
+                hipGraph_t gr = nullptr;
+                HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int const r = step(s);
+                hipError_t const e = hipStreamEndCapture(s, &gr);
+                if (r) { if (gr) (void)hipGraphDestroy(gr); return r; }
+                if (e != hipSuccess) { err = std::string("graph capture: ") + hipGetErrorString(e); return GFA_ERR_DEVICE; }
+                hipError_t const e2 = hipGraphInstantiate(&st.graph, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (e2 != hipSuccess) { st.graph = nullptr; err = std::string("graph instantiate: ") + hipGetErrorString(e2); return GFA_ERR_DEVICE; }
+                st.graph_fp = fp;
+                st.graph_seed = key;
+            }
+            for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(st.graph, s));
+        }
+        m += halfsteps_per_step * nsteps;
+        res.actions += halfsteps_per_step * nsteps;
+        res.halfsteps = m;
+        HIP_TRY(hipMemcpyAsync(st.S_host, st.S + size_t(STATUS) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        std::vector<uint8_t> act = st.active;
+        bool any = false, changed = false;
+        for (uint32_t c = 0; c < st.nCols; ++c) {
+            if (!act[c]) continue;
+            bool live = false;
+            for (int j = 0; j < 64 && !live; ++j) live = (st.S_host[64 * c + j] == ACTIVE);
+            if (live) any = true;
+            else { act[c] = 0; changed = true; }
+        }
+        if (!any) break;
+        if (changed) {
+            if (int r = set_active(plan, st, act, err)) return r;
+            scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
+        }
+    }
+    {   // per-column half-step counts of this run (columns of inactive groups count 0)
+        std::vector<double> ms(st.ncol);
+        HIP_TRY(hipMemcpyAsync(ms.data(), st.S + size_t(MSTEP) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (uint32_t q = 0; q < st.ncol; ++q) st.col_halfsteps[q] += ms[q];
+    }
+    return 0;
+}
+
 // TFQMR on T vectors for the right-hand side rhs (nullptr: unit columns), accumulating into st.x64.
 // Column groups whose columns all stopped are retired from the active set as the solve goes.
 template <typename T>
@@ -810,57 +1068,58 @@ int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double 
         if (int r = capi_autotune(plan, fp, V.Aodd, V.u, s, 2, nullptr, err)) return r;
     }
 
-    int const steps_per_check = std::max(1, check_every / 2);
-    int m = 0;
-    while (m < maxit) {
-        int const nsteps = std::min(steps_per_check, (maxit - m + 1) / 2);
-        if (eager) {
-            for (int k = 0; k < nsteps; ++k)
-                if (int r = enqueue_step<T>(plan, st, V, seed, s, err)) return r;
-        } else {
-            if (!st.graph || st.graph_fp != fp || st.graph_seed != seed) {
-                drop_graph(st);
-                hipGraph_t gr = nullptr;
-                HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                int const r = enqueue_step<T>(plan, st, V, seed, s, err);
-                hipError_t const e = hipStreamEndCapture(s, &gr);
-                if (r) { if (gr) (void)hipGraphDestroy(gr); return r; }
-                if (e != hipSuccess) { err = std::string("graph capture: ") + hipGetErrorString(e); return GFA_ERR_DEVICE; }
-                hipError_t const e2 = hipGraphInstantiate(&st.graph, gr, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(gr);
-                if (e2 != hipSuccess) { st.graph = nullptr; err = std::string("graph instantiate: ") + hipGetErrorString(e2); return GFA_ERR_DEVICE; }
-                st.graph_fp = fp;
-                st.graph_seed = seed;
-            }
-            for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(st.graph, s));
-        }
-        m += 2 * nsteps;
-        res.actions += 2 * nsteps;
-        res.halfsteps = m;
-        HIP_TRY(hipMemcpyAsync(st.S_host, st.S + size_t(STATUS) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        std::vector<uint8_t> act = st.active;
-        bool any = false, changed = false;
-        for (uint32_t c = 0; c < st.nCols; ++c) {
-            if (!act[c]) continue;
-            bool live = false;
-            for (int j = 0; j < 64 && !live; ++j) live = (st.S_host[64 * c + j] == ACTIVE);
-            if (live) any = true;
-            else { act[c] = 0; changed = true; }
-        }
-        if (!any) break;
-        if (changed) {
-            if (int r = set_active(plan, st, act, err)) return r;
-            scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
-        }
-    }
-    {   // per-column half-step counts of this run (columns of inactive groups count 0)
-        std::vector<double> ms(st.ncol);
-        HIP_TRY(hipMemcpyAsync(ms.data(), st.S + size_t(MSTEP) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        for (uint32_t q = 0; q < st.ncol; ++q) st.col_halfsteps[q] += ms[q];
-    }
+    return iterate(plan, st, scope, maxit, std::max(1, check_every / 2), 2, fp, seed * 2 + 0, eager,
+                   [&](hipStream_t ss) { return enqueue_step<T>(plan, st, V, seed, ss, err); }, res, err);
+}
+
+// one COCG iteration: q = A p, alpha, x and r, beta, p
+template <typename T>
+int enqueue_cocg(gfa_plan* plan, SolverState& st, T* r, T* p, T* q, hipStream_t s, std::string& err) {
+    int const fp = sizeof(T) == 8 ? 64 : 32;
+    SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+    dim3 const gb(st.nseg), tb(256), gs(st.nCols), ts(64 * FOLD_P);
+    if (int e = capi_action(plan, fp, q, p, s, err)) return e;
+    hipLaunchKernelGGL((k_cocg_pq<T>), gb, tb, 0, s, p, q, g);
+    hipLaunchKernelGGL(k_cocg_alpha, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
+    hipLaunchKernelGGL((k_cocg_xr<T>), gb, tb, 0, s, st.x64, r, p, q, st.S, st.ncol, g);
+    hipLaunchKernelGGL(k_cocg_beta, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
+    hipLaunchKernelGGL((k_cocg_p<T>), gb, tb, 0, s, p, r, st.S, st.ncol, g);
+    HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// COCG on T vectors for the right-hand side rhs (nullptr: unit columns), accumulating into st.x64; converges on
+// the recursive residual |r| <= max(tol_rel |b|, tol_abs).  Requires a complex-symmetric operator.
+template <typename T>
+int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
+         int check_every, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
+    if (int s = ensure_work<T>(st, err)) return s;
+    T *r = work<T>(st, 2), *p = work<T>(st, 3), *q = work<T>(st, 4);   // 0, 1: the FP64 residual's storage
+    int const fp = sizeof(T) == 8 ? 64 : 32;
+    hipStream_t const s = st.stream;
+    SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+    DevicePlan& dev = capi_dev(plan);
+    ActiveScope scope(dev, st);
+    if (st.nseg == 0) return 0;
+    hipLaunchKernelGGL((k_cocg_init<T>), dim3(st.nseg), dim3(256), 0, s, r, p, rhs, st.diag, g);
+    hipLaunchKernelGGL(k_cocg_scalar_init, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.S, st.segpart, st.col_seg,
+                       st.col_active, st.ncol, tol_rel, tol_abs);
+    HIP_TRY(hipGetLastError());
+    if (autotune && !dev.tuned[sizeof(T) == 8 ? 0 : 1]) {
+        drop_graph(st);
+        if (int e = capi_action(plan, fp, q, p, s, err)) return e;   // allocates the action's workspace
+        if (int e = capi_autotune(plan, fp, q, p, s, 2, nullptr, err)) return e;
+    }
+    return iterate(plan, st, scope, maxit, std::max(1, check_every), 1, fp, 1, eager,
+                   [&](hipStream_t ss) { return enqueue_cocg<T>(plan, st, r, p, q, ss, err); }, res, err);
+}
+
+// the inner solver of `method` (0 TFQMR, 1 COCG)
+template <typename T>
+int krylov(int method, gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
+           int check_every, uint64_t seed, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
+    if (method == 1) return cocg<T>(plan, st, rhs, tol_rel, tol_abs, maxit, check_every, eager, autotune, res, err);
+    return tfqmr<T>(plan, st, rhs, tol_rel, tol_abs, maxit, check_every, seed, eager, autotune, res, err);
 }
 
 // true residual r = b - A x64 with the FP64 action over the active groups; norms into st.res;
@@ -901,6 +1160,11 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     if (o.fp_bits != 64 && o.fp_bits != 32) { err = "fp_bits must be 32 or 64"; return GFA_ERR_ARGUMENT; }
     if (!(o.tolerance > 0)) { err = "tolerance must be positive"; return GFA_ERR_ARGUMENT; }
     if (o.max_iterations < 0 || o.max_refinements < 0) { err = "negative iteration limits"; return GFA_ERR_ARGUMENT; }
+    if (o.method != 0 && o.method != 1) { err = "method must be 0 (tfQMR) or 1 (COCG)"; return GFA_ERR_ARGUMENT; }
+    if (o.method == 1 && !capi_host(plan).complex_symmetric()) {
+        err = "COCG needs a complex-symmetric operator (real Bloch phases: Gamma, k = 1/2 or a wrapped cell)";
+        return GFA_ERR_ARGUMENT;
+    }
     auto const t0 = std::chrono::steady_clock::now();
     SolverState& st = *state_of(plan, true);
     if (int s = setup(plan, st, err)) return s;
@@ -944,7 +1208,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         if (int r = ensure_work<double>(st, err)) return r;
         double* y64 = work<double>(st, 0);  // the work vectors are free between TFQMR runs
         TfqmrResult tr;
-        if (int r = tfqmr<double>(plan, st, nullptr, o.tolerance, nullptr, o.max_iterations, check_every, seed, eager, o.autotune != 0, tr, err)) return r;
+        if (int r = krylov<double>(o.method, plan, st, nullptr, o.tolerance, nullptr, o.max_iterations, check_every, seed, eager, o.autotune != 0, tr, err)) return r;
         S.iterations += tr.halfsteps; S.actions += tr.actions;
         if (int r = set_active(plan, st, all, err)) return r;
         if (!st.rhs64 && o.max_refinements > 0) { if (int r = walloc(st, st.rhs64, st.nnzb * NBLK, err)) return r; }
@@ -956,7 +1220,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             auto const act = unconverged();
             if (int r = set_active(plan, st, act, err)) return r;
             TfqmrResult t2;
-            if (int r = tfqmr<double>(plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every, seed + k + 1, eager, false, t2, err)) return r;
+            if (int r = krylov<double>(o.method, plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every, seed + k + 1, eager, false, t2, err)) return r;
             S.iterations += t2.halfsteps; S.actions += t2.actions; S.refinements++;
             if (int r = set_active(plan, st, act, err)) return r;
             if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
@@ -986,7 +1250,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             if (any32) {
                 if (int r = set_active(plan, st, a32, err)) return r;
                 TfqmrResult tr;
-                if (int r = tfqmr<float>(plan, st, first32 ? nullptr : st.rhs32, floor_rel, st.tol_abs, budget32,
+                if (int r = krylov<float>(o.method, plan, st, first32 ? nullptr : st.rhs32, floor_rel, st.tol_abs, budget32,
                                          check_every, seed + k, eager, o.autotune != 0, tr, err)) return r;
                 first32 = false;
                 S.iterations += tr.halfsteps; S.actions += tr.actions;
@@ -1024,7 +1288,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
                 for (uint32_t c = 0; c < st.nCols; ++c) b64[c] = (!done[c] && prec64[c]) ? 1 : 0;
                 if (int r = set_active(plan, st, b64, err)) return r;
                 TfqmrResult t2;
-                if (int r = tfqmr<double>(plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every,
+                if (int r = krylov<double>(o.method, plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every,
                                           seed + 1000 + k, eager, o.autotune != 0, t2, err)) return r;
                 S.iterations += t2.halfsteps; S.actions += t2.actions;
                 if (int r = set_active(plan, st, b64, err)) return r;

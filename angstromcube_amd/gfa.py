@@ -53,7 +53,8 @@ class SolveOpts(C.Structure):
                 ("max_refinements", C.c_int32), ("check_every", C.c_int32), ("shadow_seed", C.c_uint64),
                 ("X_device", C.c_void_p), ("stream", C.c_void_p), ("inner_tolerance", C.c_double),
                 ("echo", C.c_int32), ("eager", C.c_int32), ("autotune", C.c_int32),
-                ("fp32_max_iterations", C.c_int32), ("residuals", C.POINTER(C.c_double)), ("halfsteps", C.POINTER(C.c_int32))]
+                ("fp32_max_iterations", C.c_int32), ("residuals", C.POINTER(C.c_double)), ("halfsteps", C.POINTER(C.c_int32)),
+                ("method", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -301,7 +302,7 @@ class GreenPlan:
         _check(lib().gfa_set_tile_shape(self.h, fp_bits, kind, columns, lines))
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
-              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0):
+              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0, method="tfqmr"):
         """Solve (H - E S) X = B; returns (rho[nCols, 64] complex, stats dict)."""
         import torch
         o = SolveOpts()
@@ -316,6 +317,7 @@ class GreenPlan:
         o.eager = 1 if eager else 0
         o.autotune = 1 if autotune else 0
         o.fp32_max_iterations = fp32_max_iterations
+        o.method = {"tfqmr": 0, "cocg": 1}[method]
         o.X_device = C.c_void_p(X.data_ptr()) if X is not None else None
         o.stream = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         rho = np.zeros((self.nCols, 64, 2))

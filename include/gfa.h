@@ -172,7 +172,12 @@ typedef struct {
                                  * group whose FP32 solve reduces its residual less than tenfold is solved again
                                  * in FP64 from zero                                                          */
     double* residuals;          /* optional host output [64 nCols]: final relative residual of every column          */
-    int32_t* halfsteps;         /* optional host output [64 nCols]: tfQMR half-steps of every column (all solves)     */
+    int32_t* halfsteps;         /* optional host output [64 nCols]: operator applications of every column (all solves) */
+    int32_t method;             /* inner Krylov method: 0 tfQMR (as the reference), 1 COCG (conjugate orthogonal CG;
+                                 * needs a complex-symmetric operator: real Bloch phases, e.g. Gamma or a wrapped
+                                 * cell; one application per iteration, no squared residual polynomial, so its FP32
+                                 * recurrences keep converging where tfQMR's stall)                                 */
+    int32_t reserved;
 } gfa_solve_opts;
 
 typedef struct {

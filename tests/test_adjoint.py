@@ -174,7 +174,7 @@ def test_autograd_equals_central_differences(device):
     plan = small_plan(sysm)
     V = torch.from_numpy(sysm.V.copy()).requires_grad_(True)
     hs = torch.from_numpy(A.flatten_hs(sysm.hs)).requires_grad_(True)
-    L = A.contour_trace(V, hs, plan, E2, W2, tol=1e-12)
+    L = A.contour_trace(V, hs, plan, E2, W2, tol=1e-13)
     gV, ghs = torch.autograd.grad(L, (V, hs))
     rng = np.random.default_rng(7)
     step = 1e-6
@@ -182,7 +182,7 @@ def test_autograd_equals_central_differences(device):
     ents = [int(rng.integers(hs.numel())) for _ in range(4)]
     with torch.no_grad():
         for p in pts:
-            Lp, Lm = [float(A.contour_trace(V + sgn * step * _unit(V, p), hs, plan, E2, W2, tol=1e-12))
+            Lp, Lm = [float(A.contour_trace(V + sgn * step * _unit(V, p), hs, plan, E2, W2, tol=1e-13))
                       for sgn in (+1, -1)]
             fd = (Lp - Lm) / (2 * step)
             print(f"V{p}: adjoint {float(gV[p]):.8e} fd {fd:.8e}")
@@ -190,7 +190,7 @@ def test_autograd_equals_central_differences(device):
         for e in ents:
             u = torch.zeros_like(hs)
             u[e] = 1.0
-            Lp, Lm = [float(A.contour_trace(V, hs + sgn * step * u, plan, E2, W2, tol=1e-12)) for sgn in (+1, -1)]
+            Lp, Lm = [float(A.contour_trace(V, hs + sgn * step * u, plan, E2, W2, tol=1e-13)) for sgn in (+1, -1)]
             fd = (Lp - Lm) / (2 * step)
             print(f"hs[{e}]: adjoint {float(ghs[e]):.8e} fd {fd:.8e}")
             assert abs(fd - float(ghs[e])) <= 1e-6 * float(ghs.abs().max())

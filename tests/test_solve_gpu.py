@@ -103,3 +103,31 @@ def test_mixed_solve_reaches_fp64_tolerance(device, name, mk, kw, E, k):
     assert s32["converged_columns"] == s32["total_columns"]
     assert r.max() <= 1e-9
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("name,mk,kw,E,k", [c for c in CASES if c[4] == (0.0, 0.0, 0.0)])
+def test_cocg_solves_complex_symmetric_systems(device, name, mk, kw, E, k):
+    """COCG (Gamma: A^T = A) in FP64 and in FP32 with FP64 refinement reaches the tolerance, measured with the
+    oracle's action, and gives the tfQMR solution."""
+    plan = G.GreenPlan(mk(), **kw)
+    plan.set_inputs(E, k)
+    X64, _, s64 = run(plan, device, fp_bits=64, tolerance=1e-9, max_iterations=2000, max_refinements=4)
+    ora = O.Oracle(plan, E, k)
+    for fp in (64, 32):
+        Xc, _, sc = run(plan, device, fp_bits=fp, tolerance=1e-9, max_iterations=2000, max_refinements=6,
+                        method="cocg")
+        r = true_residuals(plan, ora, Xc)
+        err = np.abs(Xc - X64).max() / np.abs(X64).max()
+        print(f"{name} cocg fp{fp}: {sc['iterations']} applications (tfqmr fp64 {s64['iterations']}), "
+              f"residual {r.max():.2e}, |X - X_tfqmr| / |X| = {err:.2e}")
+        assert sc["converged_columns"] == sc["total_columns"]
+        assert r.max() <= 1e-9
+        assert err < 1e-6
+
+
+def test_cocg_rejects_complex_phases(device):
+    name, mk, kw, E, k = CASES[1]     # k-point with complex Bloch phases
+    plan = G.GreenPlan(mk(), **kw)
+    plan.set_inputs(E, k)
+    with pytest.raises(G.GfaError, match="COCG"):
+        plan.solve(fp_bits=64, tolerance=1e-9, max_iterations=100, method="cocg")
