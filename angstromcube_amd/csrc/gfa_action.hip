@@ -1107,9 +1107,18 @@ template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
                   std::string& err, int parts) {
     auto& ws = workspace<real_t>(d);
+    // every buffer of this precision is allocated on the first call, so that later calls (also under graph
+    // capture) allocate nothing; parts = 0 only allocates
     if (d.nAtomImages > 0 && !ws.dcoef) {
         if (int r = dalloc(d, ws.partial, size_t(d.total_chunk_coefs) * 128, err)) return r;
         if (int r = dalloc(d, ws.dcoef, size_t(d.natomcoeffs) * d.nCols * 128, err)) return r;
+    }
+    if (d.nAtomImages > 0 && d.nchunks > 0 && d.max_atom_nc > 64 && !d.cscratch) {
+        if (int r = dalloc(d, d.cscratch, size_t(d.ncoeffs) * d.nCols * 128, err)) return r;
+    }
+    if (!ws.zero) {
+        if (int r = dalloc(d, ws.zero, NBLK, err)) return r;
+        HIP_TRY(hipMemsetAsync(ws.zero, 0, NBLK * sizeof(real_t), stream));
     }
     if ((parts & ACTION_PROJECT) && d.nAtomImages > 0 && d.nchunks > 0) {
         for (int L = 0; L <= 7; ++L) {
@@ -1128,9 +1137,6 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
 #undef LAUNCH_PRJ
             HIP_TRY(hipGetLastError());
         }
-        if (d.max_atom_nc > 64 && !d.cscratch) {
-            if (int r = dalloc(d, d.cscratch, size_t(d.ncoeffs) * d.nCols * 128, err)) return r;
-        }
         size_t const lds = size_t(2) * std::min<uint32_t>(d.max_atom_nc, 64) * 64 * sizeof(double);
         hipLaunchKernelGGL((k_sho_atoms<real_t>), dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), lds, stream, ws.dcoef,
                            ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.image_starts,
@@ -1144,10 +1150,6 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     a.add_start = d.add_start; a.add_index = d.add_index; a.colindx = d.colindx; a.rowindx = d.rowindx;
     a.row_pos = d.row_pos; a.image_pos = d.image_pos; a.image_lmax = d.image_lmax; a.image_starts = d.image_starts;
     a.nnzb = d.nnzb; a.nCols = d.nCols;
-    if (!ws.zero) {
-        if (int r = dalloc(d, ws.zero, NBLK, err)) return r;
-        HIP_TRY(hipMemsetAsync(ws.zero, 0, NBLK * sizeof(real_t), stream));
-    }
     a.zero = ws.zero;
     a.order = d.active_order ? d.active_order : d.order;
     a.nitems = d.active_order ? d.n_active : d.nnzb;

@@ -528,6 +528,36 @@ __global__ __launch_bounds__(256) void k_cocg_p(T* __restrict__ p, T const* __re
     }
 }
 
+// residual replacement: r = b - y with y = A x (FP64), b the unit columns; partials |r|^2, Re (r, r), Im (r, r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cocg_replace(T* __restrict__ rv, double const* __restrict__ y,
+                                                      uint8_t const* __restrict__ diag, SegArgs g) {
+    SEG_PROLOGUE
+    double acc[3][V] = {};
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+        bool const dg = diag[blk];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            double yr[V], yi[V];
+            ldd<V>(yr, y + OFF(0, i)); ldd<V>(yi, y + OFF(1, i));
+            T orr[V], oi[V];
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                double const vr = ((dg && i == j0 + t) ? 1.0 : 0.0) - yr[t], vi = -yi[t];
+                orr[t] = T(vr); oi[t] = T(vi);
+                double const cr = orr[t], ci = oi[t];
+                acc[0][t] += cr * cr + ci * ci;
+                acc[1][t] += cr * cr - ci * ci;
+                acc[2][t] += 2 * cr * ci;
+            }
+            st(rv + OFF(0, i), orr); st(rv + OFF(1, i), oi);
+        }
+    }
+    store_partials<T, 3>(acc, g.segpart, s);
+}
+
 // ------------------------------------------------------------------------------------------------
 // per-column scalar updates; partials are folded over the column group's segments in a fixed order
 // ------------------------------------------------------------------------------------------------
@@ -761,6 +791,22 @@ __global__ __launch_bounds__(64 * FOLD_P) void k_cocg_beta(double* __restrict__ 
     F(S, ncol, RHO_IM, q) = t[2];
 }
 
+// COCG after a residual replacement: rho = (r, r) of the replaced residual; convergence on its norm
+__global__ __launch_bounds__(64 * FOLD_P) void k_cocg_rho(double* __restrict__ S, double const* __restrict__ segpart,
+                                                       uint32_t const* __restrict__ col_seg,
+                                                       uint8_t const* __restrict__ col_active, uint32_t ncol) {
+    uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
+    if (!col_active[c]) return;
+    int const f[3] = {0, 1, 2};
+    double t[3];
+    fold_group<3>(segpart, col_seg, c, f, t);
+    if (threadIdx.x >= 64) return;
+    if (F(S, ncol, STATUS, q) != ACTIVE) return;
+    F(S, ncol, RHO_RE, q) = t[1];
+    F(S, ncol, RHO_IM, q) = t[2];
+    if (sqrt(t[0]) <= F(S, ncol, TOL, q)) { F(S, ncol, STATUS, q) = CONVERGED; freeze_step(S, ncol, q); }
+}
+
 // residual norms of the active column groups (others keep their value)
 __global__ __launch_bounds__(64 * FOLD_P) void k_col_norm(double* __restrict__ res, double const* __restrict__ segpart,
                                                        uint32_t const* __restrict__ col_seg,
@@ -984,9 +1030,11 @@ int enqueue_step(gfa_plan* plan, SolverState& st, Vecs<T> const& V, uint64_t see
 // Runs steps (each `halfsteps_per_step` half-steps = actions) until every column stopped or maxit half-steps:
 // replays a captured HIP graph of one step (eager: plain launches), polls the column status every
 // `steps_per_check` steps, retires finished column groups and re-captures for the smaller active set.
-template <typename StepFn>
+struct NoHook { int operator()(int) { return 0; } };
+
+template <typename StepFn, typename HookFn = NoHook>
 int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int steps_per_check, int halfsteps_per_step,
-            int fp, uint64_t key, bool eager, StepFn&& step, TfqmrResult& res, std::string& err) {
+            int fp, uint64_t key, bool eager, StepFn&& step, TfqmrResult& res, std::string& err, HookFn hook = HookFn()) {
     hipStream_t const s = st.stream;
     int m = 0;
     while (m < maxit) {
@@ -1014,6 +1062,7 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
         m += halfsteps_per_step * nsteps;
         res.actions += halfsteps_per_step * nsteps;
         res.halfsteps = m;
+        if (int r = hook(m)) return r;   // e.g. a residual replacement, between graph replays
         HIP_TRY(hipMemcpyAsync(st.S_host, st.S + size_t(STATUS) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         std::vector<uint8_t> act = st.active;
@@ -1105,13 +1154,32 @@ int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double c
     hipLaunchKernelGGL(k_cocg_scalar_init, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.S, st.segpart, st.col_seg,
                        st.col_active, st.ncol, tol_rel, tol_abs);
     HIP_TRY(hipGetLastError());
+    if (int e = capi_action(plan, fp, q, p, s, err, 0)) return e;   // allocates the action's workspace, launches nothing
     if (autotune && !dev.tuned[sizeof(T) == 8 ? 0 : 1]) {
         drop_graph(st);
-        if (int e = capi_action(plan, fp, q, p, s, err)) return e;   // allocates the action's workspace
         if (int e = capi_autotune(plan, fp, q, p, s, 2, nullptr, err)) return e;
     }
+    // FP32: every REPLACE iterations the recursive residual is replaced by the true one, b - A x with the FP64
+    // action on the FP64 solution (reliable updates, van der Vorst & Ye, SIAM J. Sci. Comput. 22, 835 (2000)), so
+    // the FP32 rounding of the recurrences cannot accumulate into a floor of the true residual; b is the unit
+    // columns also in refinement solves, since x holds the whole solution.
+    constexpr int REPLACE = 64;
+    int last = 0;
+    double* y64 = reinterpret_cast<double*>(work<T>(st, 0));   // spans work vectors 0 and 1 for FP32
+    auto hook = [&](int m) -> int {
+        if (sizeof(T) == 8 || m - last < REPLACE) return 0;
+        last = m;
+        if (int e = capi_action(plan, 64, y64, st.x64, s, err)) return e;
+        res.actions++;
+        SegArgs const g2{st.act_seg, st.seg, st.col_blocks, st.segpart};
+        hipLaunchKernelGGL((k_cocg_replace<T>), dim3(st.nseg), dim3(256), 0, s, r, y64, st.diag, g2);
+        hipLaunchKernelGGL(k_cocg_rho, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.S, st.segpart, st.col_seg,
+                           st.col_active, st.ncol);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    };
     return iterate(plan, st, scope, maxit, std::max(1, check_every), 1, fp, 1, eager,
-                   [&](hipStream_t ss) { return enqueue_cocg<T>(plan, st, r, p, q, ss, err); }, res, err);
+                   [&](hipStream_t ss) { return enqueue_cocg<T>(plan, st, r, p, q, ss, err); }, res, err, hook);
 }
 
 // the inner solver of `method` (0 TFQMR, 1 COCG)

@@ -6,11 +6,11 @@ E = -1 Ha to E_F = 0 Ha (Im E 0.005 .. 0.3 Ha) x 4 seeded k-points, truncation r
 {6, 8, 10, 12} Bohr per instance; one source block (64 columns) per instance; 8th-order kinetic stencil.
 The cell is wrapped by every radius (2 r <= 32 Bohr), so the k-point leaves the operator unchanged, as in the
 reference.  A step solves the 32 instances of one k-point as one batched plan (column groups = instances)
-with the FP32 tfQMR inside FP64 refinement; every column is verified with the FP64 action and only columns
-whose true residual meets 1e-9 are counted.
+with FP32 COCG (the operator is complex symmetric: k is inert under Wrap) inside FP64 refinement; every column
+is verified with the FP64 action and only columns whose true residual meets 1e-9 are counted.
 
-Beside it: the plain FP64 path on the same 32 instances (one plan and solve per instance, as the reference's
-contour loop calls its solver per energy), the FP64 batched solve, the local kernel's roofline from HIP events
+Beside it: the plain FP64 path on the same 32 instances (one plan and tfQMR solve per instance, as the reference's
+contour loop calls its solver per energy), the FP64 batched tfQMR and COCG solves, the local kernel's roofline from HIP events
 on its launch stream, and the reference's own CPU action (oracle/_ref, single thread) on a bounded sample.
 """
 import argparse
@@ -35,7 +35,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=32, help="instances per batch (<= 32)")
     ap.add_argument("--maxit", type=int, default=8000)
-    ap.add_argument("--fp32-maxit", type=int, default=1000, help="half-steps allowed to one FP32 solve")
+    ap.add_argument("--fp32-maxit", type=int, default=3000, help="operator applications allowed to one FP32 solve")
+    ap.add_argument("--method", choices=["cocg", "tfqmr"], default="cocg",
+                    help="inner Krylov method of the mixed solve (the FP64 comparison always runs tfQMR)")
+    ap.add_argument("--inner-tol", type=float, default=0.0,
+                    help="residual reduction asked of one FP32 solve (0: COCG with residual replacement goes to the "
+                         "final tolerance in one solve; tfQMR uses 1e-4)")
     ap.add_argument("--action-reps", type=int, default=10)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -116,8 +121,9 @@ def main():
         log(f"# copy calibration: {x.numel() * 8} bytes read and written per copy")
         return
 
-    opts = dict(tolerance=1e-9, max_iterations=args.maxit, max_refinements=6, inner_tolerance=1e-4,
-                fp32_max_iterations=args.fp32_maxit)
+    inner = args.inner_tol if args.inner_tol > 0 else (1e-30 if args.method == "cocg" else 1e-4)
+    opts = dict(tolerance=1e-9, max_iterations=args.maxit, max_refinements=6, inner_tolerance=inner,
+                fp32_max_iterations=args.fp32_maxit, method=args.method)
     # ---- warmup (also tunes the tile shapes of both precisions) ----
     for w in range(args.warmup):
         p = batch_plan(G, sysm, batch(instances, w, n))
@@ -159,6 +165,22 @@ def main():
         log(f"# fp64 batched: {s64['converged_columns']} columns in {t_b64:.2f} s")
         fp64["batched_columns_per_s"] = round(s64["converged_columns"] / t_b64, 3)
         fp64["batched_half_steps"] = [int(h) for h in s64["column_halfsteps"].reshape(-1, 64).max(axis=1)]
+        if args.method == "cocg":   # the same batch in FP64 with the mixed path's inner method
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _, sc = plan.solve(fp_bits=64, autotune=True, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2,
+                               method="cocg")
+            t_c = time.perf_counter() - t1
+            log(f"# fp64 cocg batched: {sc['converged_columns']} columns in {t_c:.2f} s")
+            fp64["cocg_batched_columns_per_s"] = round(sc["converged_columns"] / t_c, 3)
+            # the mixed path with the reference's method (FP32 tfQMR inside FP64 refinement), same batch
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _, sm = plan.solve(fp_bits=32, autotune=True, tolerance=1e-9, max_iterations=args.maxit, max_refinements=6,
+                               inner_tolerance=1e-4, fp32_max_iterations=1000, method="tfqmr")
+            t_m = time.perf_counter() - t1
+            log(f"# mixed tfqmr batched: {sm['converged_columns']} columns in {t_m:.2f} s")
+            fp64["mixed_tfqmr_batched_columns_per_s"] = round(sm["converged_columns"] / t_m, 3)
         cols, t_seq = 0, 0.0
         for d in first:   # plan built untimed (as the batched plans), device upload and solve timed
             p1 = batch_plan(G, sysm, [d])
@@ -198,7 +220,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c32 tfQMR + f64 refinement (per-group FP64 finish)",
+        "dtype": f"c32 {args.method} + f64 refinement (per-group FP64 fallback)",
         "data": "synthetic (seeded Gaussian potential, random atom matrices, Gauss-Legendre contour)",
         "config": {"workload": "batched contour: 32^3-block cell, 128 atoms numax 5, 32 energies -1..0 Ha "
                                "(Im 0.005..0.3) x 4 k-points, radii {6,8,10,12} Bohr, one source block each",
