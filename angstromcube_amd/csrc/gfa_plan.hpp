@@ -121,11 +121,12 @@ public:
     int set_kpoint(double const k[3], std::string& err);
     void update_atom_matrices();
     std::vector<double> sho_prefactors(int ia) const;  // f_i of original atom ia, zyx order
-    // A^T = A: every Bloch phase (kinetic halos and atom images) is real
+    // A^T = A: every Bloch phase the operator uses is real (kinetic halo phases only matter in periodic
+    // directions; a wrapped or isolated direction has no periodic halo images)
     bool complex_symmetric() const {
         for (int d = 0; d < 3; ++d)
             for (int s = 0; s < 2; ++s)
-                if (phase[d][s][1] != 0) return false;
+                if (bc[d] == BC_PERIODIC && phase[d][s][1] != 0) return false;
         for (size_t i = 1; i < image_phase.size(); i += 2)
             if (image_phase[i] != 0) return false;
         return true;
