@@ -463,13 +463,12 @@ __global__ __launch_bounds__(256) void k_cocg_pq(T const* __restrict__ p, T cons
     store_partials<T, 2>(acc, g.segpart, s);
 }
 
-// x += alpha p (FP64), r -= alpha q; partials |r|^2, Re (r, r), Im (r, r)
+// r -= alpha q; partials |r|^2, Re (r, r), Im (r, r)
 template <typename T>
-__global__ __launch_bounds__(256) void k_cocg_xr(double* __restrict__ x, T* __restrict__ rv, T const* __restrict__ p,
-                                                 T const* __restrict__ q, double const* __restrict__ S, uint32_t ncol,
-                                                 SegArgs g) {
+__global__ __launch_bounds__(256) void k_cocg_r(T* __restrict__ rv, T const* __restrict__ q, double const* __restrict__ S,
+                                                uint32_t ncol, SegArgs g) {
     SEG_PROLOGUE
-    Cx<double> a[V];
+    Cx<T> a[V];
     coefs(a, S, ncol, q0, ALPHA_RE);
     double acc[3][V] = {};
     for (uint32_t b = b0; b < b1; ++b) {
@@ -477,53 +476,92 @@ __global__ __launch_bounds__(256) void k_cocg_xr(double* __restrict__ x, T* __re
 #pragma unroll 2
         for (int ps = 0; ps < M::NPASS; ++ps) {
             int const i = ps * M::RPP + r;
-            T pr[V], pi[V], qr[V], qi[V], rr[V], ri[V];
-            double xr[V], xi[V];
-            ld(pr, p + OFF(0, i)); ld(pi, p + OFF(1, i));
+            T qr[V], qi[V], rr[V], ri[V];
             ld(qr, q + OFF(0, i)); ld(qi, q + OFF(1, i));
             ld(rr, rv + OFF(0, i)); ld(ri, rv + OFF(1, i));
-            ldd<V>(xr, x + OFF(0, i)); ldd<V>(xi, x + OFF(1, i));
 #pragma unroll
             for (int t = 0; t < V; ++t) {
-                xr[t] += a[t].re * double(pr[t]) - a[t].im * double(pi[t]);
-                xi[t] += a[t].re * double(pi[t]) + a[t].im * double(pr[t]);
-                T const ar = T(a[t].re), ai = T(a[t].im);
-                rr[t] = rr[t] - (ar * qr[t] - ai * qi[t]);
-                ri[t] = ri[t] - (ar * qi[t] + ai * qr[t]);
+                rr[t] = rr[t] - (a[t].re * qr[t] - a[t].im * qi[t]);
+                ri[t] = ri[t] - (a[t].re * qi[t] + a[t].im * qr[t]);
                 double const vr = rr[t], vi = ri[t];
                 acc[0][t] += vr * vr + vi * vi;
                 acc[1][t] += vr * vr - vi * vi;
                 acc[2][t] += 2 * vr * vi;
             }
-            std_<V>(x + OFF(0, i), xr); std_<V>(x + OFF(1, i), xi);
             st(rv + OFF(0, i), rr); st(rv + OFF(1, i), ri);
         }
     }
     store_partials<T, 3>(acc, g.segpart, s);
 }
 
-// p = r + beta p
+// x += alpha p with the p of this iteration, then p = r + beta p.  x is the solution itself (FP64) or, for FP32
+// vectors, the FP32 increment since the last fold into the FP64 solution.
 template <typename T>
-__global__ __launch_bounds__(256) void k_cocg_p(T* __restrict__ p, T const* __restrict__ rv, double const* __restrict__ S,
-                                                uint32_t ncol, SegArgs g) {
+__global__ __launch_bounds__(256) void k_cocg_px(T* __restrict__ x, T* __restrict__ p, T const* __restrict__ rv,
+                                                 double const* __restrict__ S, uint32_t ncol, SegArgs g) {
     SEG_PROLOGUE
-    Cx<T> be[V];
+    Cx<T> a[V], be[V];
+    coefs(a, S, ncol, q0, ALPHA_RE);
     coefs(be, S, ncol, q0, BETA_RE);
     for (uint32_t b = b0; b < b1; ++b) {
         uint32_t const blk = g.col_blocks[b];
 #pragma unroll 2
         for (int ps = 0; ps < M::NPASS; ++ps) {
             int const i = ps * M::RPP + r;
-            T pr[V], pi[V], rr[V], ri[V];
+            T pr[V], pi[V], rr[V], ri[V], xr[V], xi[V];
             ld(pr, p + OFF(0, i)); ld(pi, p + OFF(1, i));
             ld(rr, rv + OFF(0, i)); ld(ri, rv + OFF(1, i));
+            ld(xr, x + OFF(0, i)); ld(xi, x + OFF(1, i));
 #pragma unroll
             for (int t = 0; t < V; ++t) {
+                xr[t] += a[t].re * pr[t] - a[t].im * pi[t];
+                xi[t] += a[t].re * pi[t] + a[t].im * pr[t];
                 T const nr = rr[t] + (be[t].re * pr[t] - be[t].im * pi[t]);
                 T const ni = ri[t] + (be[t].re * pi[t] + be[t].im * pr[t]);
                 pr[t] = nr; pi[t] = ni;
             }
+            st(x + OFF(0, i), xr); st(x + OFF(1, i), xi);
             st(p + OFF(0, i), pr); st(p + OFF(1, i), pi);
+        }
+    }
+}
+
+// x64 += x32, x32 = 0 (the FP32 increments of the solution since the last fold)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fold_x(double* __restrict__ x64, T* __restrict__ x32, SegArgs g) {
+    SEG_PROLOGUE
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+#pragma unroll
+            for (int pl = 0; pl < 2; ++pl) {
+                T v[V], z[V];
+                double d[V];
+                ld(v, x32 + OFF(pl, i));
+                ldd<V>(d, x64 + OFF(pl, i));
+#pragma unroll
+                for (int t = 0; t < V; ++t) { d[t] += double(v[t]); z[t] = 0; }
+                std_<V>(x64 + OFF(pl, i), d);
+                st(x32 + OFF(pl, i), z);
+            }
+        }
+    }
+}
+
+// x32 = 0 over the active segments
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_x(T* __restrict__ x32, SegArgs g) {
+    SEG_PROLOGUE
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T z[V];
+#pragma unroll
+            for (int t = 0; t < V; ++t) z[t] = 0;
+            st(x32 + OFF(0, i), z); st(x32 + OFF(1, i), z);
         }
     }
 }
@@ -1123,16 +1161,16 @@ int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double 
 
 // one COCG iteration: q = A p, alpha, x and r, beta, p
 template <typename T>
-int enqueue_cocg(gfa_plan* plan, SolverState& st, T* r, T* p, T* q, hipStream_t s, std::string& err) {
+int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p, T* q, hipStream_t s, std::string& err) {
     int const fp = sizeof(T) == 8 ? 64 : 32;
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
     dim3 const gb(st.nseg), tb(256), gs(st.nCols), ts(64 * FOLD_P);
     if (int e = capi_action(plan, fp, q, p, s, err)) return e;
     hipLaunchKernelGGL((k_cocg_pq<T>), gb, tb, 0, s, p, q, g);
     hipLaunchKernelGGL(k_cocg_alpha, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
-    hipLaunchKernelGGL((k_cocg_xr<T>), gb, tb, 0, s, st.x64, r, p, q, st.S, st.ncol, g);
+    hipLaunchKernelGGL((k_cocg_r<T>), gb, tb, 0, s, r, q, st.S, st.ncol, g);
     hipLaunchKernelGGL(k_cocg_beta, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
-    hipLaunchKernelGGL((k_cocg_p<T>), gb, tb, 0, s, p, r, st.S, st.ncol, g);
+    hipLaunchKernelGGL((k_cocg_px<T>), gb, tb, 0, s, x, p, r, st.S, st.ncol, g);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1143,32 +1181,47 @@ template <typename T>
 int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
          int check_every, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
     if (int s = ensure_work<T>(st, err)) return s;
-    T *r = work<T>(st, 2), *p = work<T>(st, 3), *q = work<T>(st, 4);   // 0, 1: the FP64 residual's storage
-    int const fp = sizeof(T) == 8 ? 64 : 32;
+    // 0, 1: the FP64 residual's storage (FP32); r, p, q; x = the FP32 increment of the solution (FP32) or the
+    // FP64 solution itself
+    T *r = work<T>(st, 2), *p = work<T>(st, 3), *q = work<T>(st, 4);
+    bool const f32 = sizeof(T) == 4;
+    T* x = f32 ? work<T>(st, 5) : reinterpret_cast<T*>(st.x64);
+    int const fp = f32 ? 32 : 64;
     hipStream_t const s = st.stream;
-    SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
     DevicePlan& dev = capi_dev(plan);
     ActiveScope scope(dev, st);
     if (st.nseg == 0) return 0;
-    hipLaunchKernelGGL((k_cocg_init<T>), dim3(st.nseg), dim3(256), 0, s, r, p, rhs, st.diag, g);
-    hipLaunchKernelGGL(k_cocg_scalar_init, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.S, st.segpart, st.col_seg,
-                       st.col_active, st.ncol, tol_rel, tol_abs);
-    HIP_TRY(hipGetLastError());
+    std::vector<uint8_t> const started = st.active;
+    {
+        SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+        hipLaunchKernelGGL((k_cocg_init<T>), dim3(st.nseg), dim3(256), 0, s, r, p, rhs, st.diag, g);
+        hipLaunchKernelGGL(k_cocg_scalar_init, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.S, st.segpart, st.col_seg,
+                           st.col_active, st.ncol, tol_rel, tol_abs);
+        if (f32) hipLaunchKernelGGL((k_zero_x<T>), dim3(st.nseg), dim3(256), 0, s, x, g);
+        HIP_TRY(hipGetLastError());
+    }
     if (int e = capi_action(plan, fp, q, p, s, err, 0)) return e;   // allocates the action's workspace, launches nothing
-    if (autotune && !dev.tuned[sizeof(T) == 8 ? 0 : 1]) {
+    if (autotune && !dev.tuned[f32 ? 1 : 0]) {
         drop_graph(st);
         if (int e = capi_autotune(plan, fp, q, p, s, 2, nullptr, err)) return e;
     }
-    // FP32: every REPLACE iterations the recursive residual is replaced by the true one, b - A x with the FP64
-    // action on the FP64 solution (reliable updates, van der Vorst & Ye, SIAM J. Sci. Comput. 22, 835 (2000)), so
-    // the FP32 rounding of the recurrences cannot accumulate into a floor of the true residual; b is the unit
-    // columns also in refinement solves, since x holds the whole solution.
+    auto fold = [&]() -> int {   // x64 += x32 over the active groups
+        SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
+        hipLaunchKernelGGL((k_fold_x<T>), dim3(st.nseg), dim3(256), 0, s, st.x64, x, g);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    };
+    // FP32: every REPLACE iterations the FP32 increments are folded into the FP64 solution and the recursive
+    // residual is replaced by the true one, b - A x with the FP64 action (reliable updates, van der Vorst & Ye,
+    // SIAM J. Sci. Comput. 22, 835 (2000)), so the FP32 rounding of the recurrences cannot accumulate into a floor
+    // of the true residual; b is the unit columns also in refinement solves, since x holds the whole solution.
     constexpr int REPLACE = 64;
     int last = 0;
     double* y64 = reinterpret_cast<double*>(work<T>(st, 0));   // spans work vectors 0 and 1 for FP32
     auto hook = [&](int m) -> int {
-        if (sizeof(T) == 8 || m - last < REPLACE) return 0;
+        if (!f32 || m - last < REPLACE) return 0;
         last = m;
+        if (int e = fold()) return e;
         if (int e = capi_action(plan, 64, y64, st.x64, s, err)) return e;
         res.actions++;
         SegArgs const g2{st.act_seg, st.seg, st.col_blocks, st.segpart};
@@ -1178,8 +1231,16 @@ int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double c
         HIP_TRY(hipGetLastError());
         return 0;
     };
-    return iterate(plan, st, scope, maxit, std::max(1, check_every), 1, fp, 1, eager,
-                   [&](hipStream_t ss) { return enqueue_cocg<T>(plan, st, r, p, q, ss, err); }, res, err, hook);
+    int const rc = iterate(plan, st, scope, maxit, std::max(1, check_every), 1, fp, 1, eager,
+                           [&](hipStream_t ss) { return enqueue_cocg<T>(plan, st, x, r, p, q, ss, err); }, res, err,
+                           hook);
+    if (rc) return rc;
+    if (f32) {   // the increments of every group that took part, also of those retired since the last fold
+        if (int e = set_active(plan, st, started, err)) return e;
+        scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
+        if (int e = fold()) return e;
+    }
+    return 0;
 }
 
 // the inner solver of `method` (0 TFQMR, 1 COCG)
