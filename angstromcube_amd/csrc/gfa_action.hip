@@ -20,6 +20,7 @@
 
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdio>
 
@@ -73,6 +74,35 @@ __device__ __forceinline__ void hermite_1d(real_t (*H)[3][4], float (*xi2)[4], i
     }
 }
 
+// The Hermite functions of every (atom image, target row) pair that the projection and the add-back use, computed
+// once per plan with hermite_1d itself (so the values are those the kernels computed on the fly before):
+// htab[slot][8][3][4] (FP64, numax <= 7) and xtab[slot][3][4] = xi^2 (float).
+__global__ __launch_bounds__(64) void k_hermite_table(double* __restrict__ htab, float* __restrict__ xtab,
+                                                      uint32_t const* __restrict__ slot_img,
+                                                      uint32_t const* __restrict__ slot_row,
+                                                      double const* __restrict__ image_pos,
+                                                      int8_t const* __restrict__ image_lmax,
+                                                      float const* __restrict__ row_pos, double hx, double hy,
+                                                      double hz, double rproj) {
+    uint32_t const sl = blockIdx.x;
+    int const t = threadIdx.x;
+    if (t >= 12) return;
+    uint32_t const img = slot_img[sl], row = slot_row[sl];
+    double const hgrid[4] = {hx, hy, hz, rproj};
+    hermite_1d<double>(reinterpret_cast<double (*)[3][4]>(htab + size_t(sl) * 96),
+                       reinterpret_cast<float (*)[4]>(xtab + size_t(sl) * 12), t, image_lmax[img],
+                       image_pos + 4 * size_t(img), row_pos + 4 * size_t(row), hgrid);
+}
+
+// stage the table entry of `slot` into the LDS arrays H[L+1][3][4] (real_t) and xi2[3][4]; no barrier
+template <typename real_t>
+__device__ __forceinline__ void stage_hermite(real_t (*H)[3][4], float (*xi2)[4], double const* __restrict__ htab,
+                                              float const* __restrict__ xtab, uint32_t slot, int L, int tid, int T) {
+    double const* src = htab + size_t(slot) * 96;
+    for (int e = tid; e < (L + 1) * 12; e += T) H[e / 12][(e % 12) / 4][e % 4] = real_t(src[e]);
+    if (tid < 12) xi2[tid / 4][tid % 4] = xtab[size_t(slot) * 12 + tid];
+}
+
 // ---------------------------------------------------------------------------------------------
 // 1. projection: one work item = (atom image, column, chunk of blocks); 128 threads = (j, plane)
 // ---------------------------------------------------------------------------------------------
@@ -82,18 +112,15 @@ __global__ __launch_bounds__(128) void k_sho_project(
     uint32_t const* __restrict__ bucket, uint32_t const* __restrict__ chunk_coef,
     uint32_t const* __restrict__ prj_blocks, uint32_t const* __restrict__ rowindx,
     float const* __restrict__ row_pos, double const* __restrict__ image_pos, double hx, double hy, double hz,
-    double rproj, uint8_t const* __restrict__ col_active) {
+    double rproj, uint8_t const* __restrict__ col_active, double const* __restrict__ htab,
+    float const* __restrict__ xtab, uint32_t const* __restrict__ prj_slot) {
     __shared__ real_t H[L + 1][3][4];
     __shared__ float xi2[3][4];
-    __shared__ double apos[4];
-    __shared__ float cpos[4];
-    __shared__ double hgrid[4];
     uint32_t const ich = bucket[blockIdx.x];
     ProjChunk const ch = chunks[ich];
     if (col_active && !col_active[ch.col]) return;  // a retired column group
     int const tid = threadIdx.x, j = tid & 63, p = tid >> 6;
-    if (tid < 4) apos[tid] = image_pos[4 * size_t(ch.image) + tid];
-    if (tid == 0) { hgrid[0] = hx; hgrid[1] = hy; hgrid[2] = hz; hgrid[3] = rproj; }
+    (void)rowindx; (void)row_pos; (void)image_pos; (void)hx; (void)hy; (void)hz;
     constexpr int n3 = Sho<L>::n3, n2 = Sho<L>::n2;
     real_t czyx[n3];
 #pragma unroll
@@ -102,9 +129,7 @@ __global__ __launch_bounds__(128) void k_sho_project(
     for (uint32_t b = ch.begin; b < ch.end; ++b) {
         uint32_t const inzb = prj_blocks[b];
         __syncthreads();
-        if (tid < 3) cpos[tid] = row_pos[4 * size_t(rowindx[inzb]) + tid];
-        __syncthreads();
-        if (tid < 12) hermite_1d<real_t>(H, xi2, tid, L, apos, cpos, hgrid);
+        stage_hermite<real_t>(H, xi2, htab, xtab, prj_slot[b], L, tid, 128);
         __syncthreads();
         real_t const* xb = x + size_t(inzb) * NBLK + p * 4096 + j;
 #pragma unroll
@@ -180,8 +205,20 @@ __global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
     int const j = threadIdx.x & 63, w = threadIdx.x >> 6;
     int const L = atom_lmax[a];
     int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
-    double* cre = (nc <= 64) ? lds : cscratch + (size_t(atom_starts[a]) * nCols + size_t(c) * nc) * 128;
+    bool const in_lds = nc <= 64;
+    double* cre = in_lds ? lds : cscratch + (size_t(atom_starts[a]) * nCols + size_t(c) * nc) * 128;
     double* cim = cre + nc * 64;
+    // M = (h - E_c s) dVol of this column's energy, staged in LDS (after the coefficients) for broadcast reads
+    double* Mre = in_lds ? lds + 2 * nc * 64 : nullptr;
+    double* Mim = in_lds ? Mre + nc * nc : nullptr;
+    double const* HS = atom_mat + atom_mat_off[a];  // [2][nc][nc]: h dVol, s dVol
+    double const Er = E_col[2 * c], Ei = E_col[2 * c + 1];
+    if (in_lds)
+        for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
+            double const hd = HS[e], sd = HS[nc * nc + e];
+            Mre[e] = hd - Er * sd;
+            Mim[e] = -Ei * sd;
+        }
     uint32_t const k0 = sum_start[a], k1 = sum_start[a + 1];
     for (int s = w; s < nc; s += ATOM_WAVES) {
         double ar = 0, ai = 0;
@@ -202,15 +239,17 @@ __global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
         cim[s * 64 + j] = ai;
     }
     __syncthreads();
-    double const* HS = atom_mat + atom_mat_off[a];  // [2][nc][nc]: h dVol, s dVol
-    double const Er = E_col[2 * c], Ei = E_col[2 * c + 1];
     // d = M c with M = (h - E s) dVol of this column's energy, complex double accumulation in ascending
     // column order (include/green_dyadic.hxx:800-819, src/green_function.cxx:97-98)
     for (int i = w; i < nc; i += ATOM_WAVES) {
         double dr = 0, di = 0;
         for (int k = 0; k < nc; ++k) {
-            double const hd = HS[i * nc + k], sd = HS[nc * nc + i * nc + k];
-            double const mr = hd - Er * sd, mi = -Ei * sd;
+            double mr, mi;
+            if (in_lds) { mr = Mre[i * nc + k]; mi = Mim[i * nc + k]; }
+            else {
+                double const hd = HS[i * nc + k], sd = HS[nc * nc + i * nc + k];
+                mr = hd - Er * sd; mi = -Ei * sd;
+            }
             double const cr = cre[k * 64 + j], ci = cim[k * 64 + j];
             dr += mr * cr - mi * ci;
             di += mr * ci + mi * cr;
@@ -254,6 +293,9 @@ struct LocalArgs {
     uint32_t nCols;
     uint32_t per_xcd;        // work-items per XCD in the XCD-contiguous mapping
     double const* E_col;     // [nCols][2] energy of each column group
+    double const* htab;      // Hermite functions per (image, row) slot
+    float const* xtab;
+    uint32_t const* add_slot;  // slot of each add-list entry
     real_t c8[3][5];         // 8th-order weights per direction
     real_t c16[3][9];        // 16th-order weights per direction
     int nh[3];               // neighbour blocks per side used in direction d (1 or 2)
@@ -329,9 +371,6 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
     __shared__ real_t ys[2 * 64 * JW];
     __shared__ real_t H[8][3][4];
     __shared__ float xi2[3][4];
-    __shared__ double apos[4];
-    __shared__ float cpos[4];
-    __shared__ double hgrid[4];
 
     uint64_t lin = blockIdx.x;
     if (a.per_xcd) lin = uint64_t(blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
@@ -354,7 +393,6 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
             *reinterpret_cast<vec_t*>(xs + row * JW + cv * VE) = *reinterpret_cast<vec_t const*>(src + row * 64 + cv * VE);
         }
     }
-    if (tid < 4) hgrid[tid] = a.hgrid[tid];
     __syncthreads();
 
     // ---- potential: ys = (V - E) x ----
@@ -463,8 +501,7 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
             if (L > 5) continue;  // done by k_sho_add_wide
             int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
             __syncthreads();
-            if (tid < 4) apos[tid] = a.image_pos[4 * size_t(img) + tid];
-            if (tid < 3) cpos[tid] = a.row_pos[4 * size_t(a.rowindx[blk]) + tid];
+            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
             {
                 real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
                 for (int v = tid; v < 2 * nc * VPR; v += T) {
@@ -473,8 +510,6 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
                         *reinterpret_cast<vec_t const*>(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * VE);
                 }
             }
-            __syncthreads();
-            if (tid < 12) hermite_1d<real_t>(H, xi2, tid, L, apos, cpos, hgrid);
             __syncthreads();
             sho_add_atom<real_t, JW>(ysp, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
         }
@@ -616,9 +651,6 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     __shared__ real_t xs[2 * 64 * JW];
     __shared__ real_t H[8][3][4];
     __shared__ float xi2[3][4];
-    __shared__ double apos[4];
-    __shared__ float cpos[4];
-    __shared__ double hgrid[4];
 
     uint64_t lin = blockIdx.x;
     if (a.per_xcd) lin = uint64_t(blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
@@ -640,7 +672,6 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
             *reinterpret_cast<vec_t*>(xs + row * JW + cv * VE) = *reinterpret_cast<vec_t const*>(src + row * 64 + cv * VE);
         }
     }
-    if (tid < 4) hgrid[tid] = a.hgrid[tid];
     __syncthreads();
 
     int const jl = tid % JW, p = (tid / JW) & 1, zq = tid / (2 * JW);
@@ -726,8 +757,7 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
             if (L > 5) continue;  // done by k_sho_add_wide
             int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
             __syncthreads();
-            if (tid < 4) apos[tid] = a.image_pos[4 * size_t(img) + tid];
-            if (tid < 3) cpos[tid] = a.row_pos[4 * size_t(a.rowindx[blk]) + tid];
+            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
             {
                 real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
                 for (int v = tid; v < 2 * nc * VPR; v += T) {
@@ -736,8 +766,6 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
                         *reinterpret_cast<vec_t const*>(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * VE);
                 }
             }
-            __syncthreads();
-            if (tid < 12) hermite_1d<real_t>(H, xi2, tid, L, apos, cpos, hgrid);
             __syncthreads();
             sho_add_regs<real_t>(acc, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
         }
@@ -908,6 +936,18 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
         d.n_wide_pairs = uint32_t(wide.size() / 2);
         if ((s = dupload(d, d.wide_pairs, wide, err))) return s;
     }
+    {   // (image, target row) slots of the Hermite table, for the add lists and the projection chunks below
+        std::map<uint64_t, uint32_t> slot_of;
+        std::vector<uint32_t> add_slot(p.sho_add.index.size());
+        for (uint64_t blk = 0; blk < p.nnzb; ++blk)
+            for (uint32_t k = p.sho_add.start[blk]; k < p.sho_add.start[blk + 1]; ++k) {
+                uint64_t const key = (uint64_t(p.sho_add.index[k]) << 32) | p.rowindx[blk];
+                auto const it = slot_of.emplace(key, uint32_t(slot_of.size())).first;
+                add_slot[k] = it->second;
+            }
+        d.slot_map = std::move(slot_of);
+        if ((s = dupload(d, d.add_slot, add_slot, err))) return s;
+    }
     if ((s = dupload(d, d.image_pos, p.image_pos, err))) return s;
     if ((s = dupload(d, d.image_lmax, p.image_lmax, err))) return s;
     if ((s = dupload(d, d.image_starts, p.image_starts, err))) return s;
@@ -953,8 +993,34 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
                 if (p.image_lmax[chunks[k].image] == L) bucket.push_back(k);
         }
         d.bucket_start[8] = uint32_t(bucket.size());
+        std::vector<uint32_t> prj_slot(blocks.size());
+        for (auto const& chk : chunks)
+            for (uint32_t b = chk.begin; b < chk.end; ++b) {
+                uint64_t const key = (uint64_t(chk.image) << 32) | p.rowindx[blocks[b]];
+                auto const it = d.slot_map.emplace(key, uint32_t(d.slot_map.size())).first;
+                prj_slot[b] = it->second;
+            }
         if ((s = dupload(d, d.chunks, chunks, err))) return s;
         if ((s = dupload(d, d.prj_blocks, blocks, err))) return s;
+        if ((s = dupload(d, d.prj_slot, prj_slot, err))) return s;
+        {   // the table itself, computed on the device by hermite_1d
+            size_t const ns = std::max<size_t>(1, d.slot_map.size());
+            std::vector<uint32_t> simg(ns, 0), srow(ns, 0);
+            for (auto const& kv : d.slot_map) { simg[kv.second] = uint32_t(kv.first >> 32); srow[kv.second] = uint32_t(kv.first & 0xffffffffu); }
+            uint32_t *dimg = nullptr, *drow = nullptr;
+            if ((s = dupload(d, dimg, simg, err))) return s;
+            if ((s = dupload(d, drow, srow, err))) return s;
+            if ((s = dalloc(d, d.htab, ns * 96, err))) return s;
+            if ((s = dalloc(d, d.xtab, ns * 12, err))) return s;
+            if (!d.slot_map.empty()) {
+                hipLaunchKernelGGL(k_hermite_table, dim3(uint32_t(d.slot_map.size())), dim3(64), 0, 0, d.htab, d.xtab,
+                                   dimg, drow, d.image_pos, d.image_lmax, d.row_pos, p.hg[0], p.hg[1], p.hg[2], p.r_proj);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipDeviceSynchronize());
+            }
+            d.nslots = uint32_t(d.slot_map.size());
+            d.slot_map.clear();
+        }
         if ((s = dupload(d, d.chunk_coef, coef, err))) return s;
         if ((s = dupload(d, d.pair_chunk, pair_chunk, err))) return s;
         if ((s = dupload(d, d.bucket_chunks, bucket, err))) return s;
@@ -1127,7 +1193,7 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
             uint32_t const* bucket = d.bucket_chunks + d.bucket_start[L];
 #define LAUNCH_PRJ(LL) hipLaunchKernelGGL((k_sho_project<real_t, LL>), dim3(n), dim3(128), 0, stream, ws.partial, x, \
                 d.chunks, bucket, d.chunk_coef, d.prj_blocks, d.rowindx, d.row_pos, d.image_pos, \
-                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], d.col_active)
+                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], d.col_active, d.htab, d.xtab, d.prj_slot)
             switch (L) {
                 case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
                 case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;
@@ -1137,7 +1203,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
 #undef LAUNCH_PRJ
             HIP_TRY(hipGetLastError());
         }
-        size_t const lds = size_t(2) * std::min<uint32_t>(d.max_atom_nc, 64) * 64 * sizeof(double);
+        uint32_t const ncl = std::min<uint32_t>(d.max_atom_nc, 64);   // larger atoms use the global scratch
+        size_t const lds = (size_t(2) * ncl * 64 + size_t(2) * ncl * ncl) * sizeof(double);
         hipLaunchKernelGGL((k_sho_atoms<real_t>), dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), lds, stream, ws.dcoef,
                            ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.image_starts,
                            d.atom_lmax, d.atom_hs, d.atom_mat_off, d.image_phase, d.nCols, d.cscratch,
@@ -1154,6 +1221,7 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     a.order = d.active_order ? d.active_order : d.order;
     a.nitems = d.active_order ? d.n_active : d.nnzb;
     a.E_col = d.E_col;
+    a.htab = d.htab; a.xtab = d.xtab; a.add_slot = d.add_slot;
     static int const c8i[5] = {-14350, 8064, -1008, 128, -9};
     static int const c16i[9] = {-924708642, 538137600, -94174080, 22830080, -5350800, 1053696, -156800, 15360, -735};
     for (int dd = 0; dd < 3; ++dd) {
@@ -1279,7 +1347,7 @@ int device_gradient(DevicePlan& d, Plan const& p, ActionScalars const& s, double
                 uint32_t const* bucket = d.bucket_chunks + d.bucket_start[L];
 #define LAUNCH_PRJ(LL) hipLaunchKernelGGL((k_sho_project<double, LL>), dim3(n), dim3(128), 0, stream, ws.partial, vec, \
                 d.chunks, bucket, d.chunk_coef, d.prj_blocks, d.rowindx, d.row_pos, d.image_pos, \
-                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], (uint8_t const*)nullptr)
+                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], (uint8_t const*)nullptr, d.htab, d.xtab, d.prj_slot)
                 switch (L) {
                     case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
                     case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,13 @@ struct DevicePlan {
     int32_t* veff = nullptr;           // [nnzb]
     int32_t* nbr = nullptr;            // [3][2][nh_max][nnzb] neighbour codes
     uint32_t* order = nullptr;         // [nnzb] processing order of k_local
+    // Hermite functions per (atom image, target row) slot: htab[slot][8][3][4], xtab[slot][3][4] = xi^2
+    double* htab = nullptr;
+    float* xtab = nullptr;
+    uint32_t* add_slot = nullptr;      // [add nnz] slot of each add-list entry
+    uint32_t* prj_slot = nullptr;      // [projection blocks] slot of each chunk block
+    uint32_t nslots = 0;
+    std::map<uint64_t, uint32_t> slot_map;  // (image << 32 | row) -> slot, during upload only
     std::vector<uint32_t> h_order;     // host copy of order
     float* row_pos = nullptr;          // [nRows][4]
     uint32_t* add_start = nullptr;     // [nnzb+1]
