@@ -44,7 +44,13 @@ namespace {
     } while (0)
 
 constexpr int NBLK = 2 * 64 * 64;
-constexpr int SEG = 16;  // blocks per segment (work-group of the vector kernels)
+#ifndef VEC_NT
+#define VEC_NT 1
+#endif
+#ifndef SEG_BLOCKS
+#define SEG_BLOCKS 16
+#endif
+constexpr int SEG = SEG_BLOCKS;  // blocks per segment (work-group of the vector kernels)
 
 // per-column scalar fields, stored field-major: S[field * ncol + q]
 enum Field { RHO_RE, RHO_IM, ALPHA_RE, ALPHA_IM, THETA, TAU, ETAE_RE, ETAE_IM, ETAO_RE, ETAO_IM,
@@ -66,9 +72,14 @@ template <typename T> struct Map {
     static constexpr int NPASS = 64 / RPP;
 };
 
+// the vectors are streamed once per kernel: non-temporal 16-byte accesses (no reuse in L2 within a kernel)
 template <typename T>
 __device__ __forceinline__ void ld(T (&a)[Pk<T>::V], T const* p) {
+#if VEC_NT
+    typename Pk<T>::t const v = __builtin_nontemporal_load(reinterpret_cast<typename Pk<T>::t const*>(p));
+#else
     typename Pk<T>::t const v = *reinterpret_cast<typename Pk<T>::t const*>(p);
+#endif
 #pragma unroll
     for (int t = 0; t < Pk<T>::V; ++t) a[t] = v[t];
 }
@@ -77,7 +88,11 @@ __device__ __forceinline__ void st(T* p, T const (&a)[Pk<T>::V]) {
     typename Pk<T>::t v;
 #pragma unroll
     for (int t = 0; t < Pk<T>::V; ++t) v[t] = a[t];
+#if VEC_NT
+    __builtin_nontemporal_store(v, reinterpret_cast<typename Pk<T>::t*>(p));
+#else
     *reinterpret_cast<typename Pk<T>::t*>(p) = v;
+#endif
 }
 // V doubles (V = 2 or 4) as 16-byte pieces
 template <int V>
