@@ -296,6 +296,7 @@ struct LocalArgs {
     double const* htab;      // Hermite functions per (image, row) slot
     float const* xtab;
     uint32_t const* add_slot;  // slot of each add-list entry
+    double* dot;             // if set: [nnzb][re, im][64] bilinear products sum_i x[b][i][j] y[b][i][j] per block
     real_t c8[3][5];         // 8th-order weights per direction
     real_t c16[3][9];        // 16th-order weights per direction
     int nh[3];               // neighbour blocks per side used in direction d (1 or 2)
@@ -525,6 +526,31 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
             *reinterpret_cast<vec_t*>(dst + row * 64 + cv * VE) = *reinterpret_cast<vec_t const*>(ys + row * JW + cv * VE);
         }
     }
+    if (a.dot) {   // (x, y) = sum_i x_i y_i (no conjugation) per column, x re-read from global memory (L2)
+        int const jl = tid % JW, grp = tid / JW;   // 8 groups of 8 points
+        int const j = slice * JW + jl;
+        real_t const* xb = a.x + blk * NBLK + j;
+        double re = 0, im = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int const i = grp * 8 + q;
+            double const yr = ys[i * JW + jl], yi = ys[(64 + i) * JW + jl];
+            double const xr = xb[i * 64], xi = xb[4096 + i * 64];
+            re += xr * yr - xi * yi;
+            im += xr * yi + xi * yr;
+        }
+        __syncthreads();   // xs is free: the partial sums of the 8 groups
+        double* red = reinterpret_cast<double*>(xs);
+        red[(grp * 2 + 0) * JW + jl] = re;
+        red[(grp * 2 + 1) * JW + jl] = im;
+        __syncthreads();
+        if (tid < JW) {
+            double sr = 0, si = 0;
+            for (int g = 0; g < 8; ++g) { sr += red[(g * 2) * JW + tid]; si += red[(g * 2 + 1) * JW + tid]; }
+            a.dot[(blk * 2 + 0) * 64 + slice * JW + tid] = sr;
+            a.dot[(blk * 2 + 1) * 64 + slice * JW + tid] = si;
+        }
+    }
 }
 
 
@@ -744,6 +770,13 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
         }
     }
 
+    // the other plane of x at this thread's points, for the bilinear product (x, y) of the epilogue
+    real_t xo[16];
+    if (a.dot) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) xo[k] = xso[(zq * 16 + k) * JW];
+    }
+
     // ---- SHO add-back of the atom images (numax <= 5) touching this block; the coefficients of each image are
     //      staged in the LDS buffer of the block slice, which is free after the kinetic passes ----
     uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
@@ -774,6 +807,28 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     real_t* dst = a.y + blk * NBLK + off_p + size_t(zq * 16) * 64;
 #pragma unroll
     for (int k = 0; k < 16; ++k) dst[size_t(k) * 64] = acc[k];
+    if (a.dot) {
+        // (x, y) = sum_i x_i y_i per column: plane 0 holds x_r, y_r (+ x_i from LDS), plane 1 x_i, y_i (+ x_r):
+        // Re = sum x_r y_r - sum x_i y_i, Im = sum x_i y_r + sum x_r y_i
+        double re = 0, im = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            re += double(own[k]) * double(acc[k]);
+            im += double(xo[k]) * double(acc[k]);
+        }
+        if (p == 1) re = -re;
+        __syncthreads();   // the LDS of the slice is free
+        double* red = reinterpret_cast<double*>(xs);   // [zq][p][re, im][JW]
+        red[((zq * 2 + p) * 2 + 0) * JW + jl] = re;
+        red[((zq * 2 + p) * 2 + 1) * JW + jl] = im;
+        __syncthreads();
+        if (tid < JW) {
+            double sr = 0, si = 0;
+            for (int g = 0; g < 8; ++g) { sr += red[(g * 2 + 0) * JW + tid]; si += red[(g * 2 + 1) * JW + tid]; }
+            a.dot[(blk * 2 + 0) * 64 + slice * JW + tid] = sr;
+            a.dot[(blk * 2 + 1) * 64 + slice * JW + tid] = si;
+        }
+    }
 }
 
 
@@ -1171,7 +1226,7 @@ template <> Workspace<float>& workspace<float>(DevicePlan& d) { return d.ws32; }
 
 template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
-                  std::string& err, int parts) {
+                  std::string& err, int parts, double* dot) {
     auto& ws = workspace<real_t>(d);
     // every buffer of this precision is allocated on the first call, so that later calls (also under graph
     // capture) allocate nothing; parts = 0 only allocates
@@ -1222,6 +1277,7 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     a.nitems = d.active_order ? d.n_active : d.nnzb;
     a.E_col = d.E_col;
     a.htab = d.htab; a.xtab = d.xtab; a.add_slot = d.add_slot;
+    a.dot = dot;
     static int const c8i[5] = {-14350, 8064, -1008, 128, -9};
     static int const c16i[9] = {-924708642, 538137600, -94174080, 22830080, -5350800, 1053696, -156800, 15360, -735};
     for (int dd = 0; dd < 3; ++dd) {
@@ -1259,8 +1315,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     return 0;
 }
 
-template int device_action<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, std::string&, int);
-template int device_action<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, std::string&, int);
+template int device_action<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, std::string&, int, double*);
+template int device_action<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, std::string&, int, double*);
 
 // Times every k_local tile shape on the caller's vectors (the whole action, reps launches each, HIP events on
 // `stream`) and keeps the fastest for this precision.  y is overwritten.

@@ -71,10 +71,11 @@ gfa::ActionScalars scalars(gfa::Plan const& h) {
 
 namespace gfa {
 // used by the solver: run one action on a plan handle
-int capi_action(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, std::string& err, int parts) {
+int capi_action(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, std::string& err, int parts,
+                double* dot) {
     auto const s = scalars(p->host);
-    if (fp_bits == 64) return device_action<double>(p->dev, s, (double*)y, (double const*)x, stream, err, parts);
-    return device_action<float>(p->dev, s, (float*)y, (float const*)x, stream, err, parts);
+    if (fp_bits == 64) return device_action<double>(p->dev, s, (double*)y, (double const*)x, stream, err, parts, dot);
+    return device_action<float>(p->dev, s, (float*)y, (float const*)x, stream, err, parts, dot);
 }
 int capi_autotune(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, int reps, double* best_ms,
                   std::string& err) {

@@ -126,7 +126,7 @@ void device_free(DevicePlan& d);
 enum { ACTION_PROJECT = 1, ACTION_LOCAL = 2, ACTION_ALL = 3 };
 template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
-                  std::string& err, int parts = ACTION_ALL);
+                  std::string& err, int parts = ACTION_ALL, double* dot = nullptr);
 
 // flop count of one action as the reference counts it (include/green_action.hxx:154-175)
 template <typename real_t>

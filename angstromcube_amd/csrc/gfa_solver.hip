@@ -541,6 +541,21 @@ __global__ __launch_bounds__(256) void k_cocg_px(T* __restrict__ x, T* __restric
     }
 }
 
+// segment partials of (p, A p) from the per-block products the action computed: segpart[s][0, 1][j]
+__global__ __launch_bounds__(64) void k_fold_blocks(double const* __restrict__ bpart, SegArgs g) {
+    uint32_t const s = g.act[blockIdx.x];
+    uint32_t const b0 = g.seg[3 * s + 1], b1 = g.seg[3 * s + 2];
+    int const j = threadIdx.x;
+    double re = 0, im = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint64_t const blk = g.col_blocks[b];
+        re += bpart[(blk * 2 + 0) * 64 + j];
+        im += bpart[(blk * 2 + 1) * 64 + j];
+    }
+    g.segpart[(uint64_t(s) * 3 + 0) * 64 + j] = re;
+    g.segpart[(uint64_t(s) * 3 + 1) * 64 + j] = im;
+}
+
 // x64 += x32, x32 = 0 (the FP32 increments of the solution since the last fold)
 template <typename T>
 __global__ __launch_bounds__(256) void k_fold_x(double* __restrict__ x64, T* __restrict__ x32, SegArgs g) {
@@ -922,6 +937,7 @@ struct SolverState {
     double* S_host = nullptr;               // pinned copy of the status field
     double* res = nullptr;                  // [ncol] true residual norms
     double* tol_abs = nullptr;              // [ncol]
+    double* bpart = nullptr;                // [nnzb][2][64] per-block (p, A p) from the action
     std::vector<double> col_halfsteps;      // [ncol] half-steps of every column in this solve
     // vectors
     double* x64 = nullptr;                  // the FP64 solution
@@ -1007,6 +1023,7 @@ int setup(gfa_plan* plan, SolverState& st, std::string& err) {
     if ((s = walloc(st, st.S, size_t(NFIELDS) * st.ncol, err))) return s;
     if ((s = walloc(st, st.res, st.ncol, err))) return s;
     if ((s = walloc(st, st.tol_abs, st.ncol, err))) return s;
+    if ((s = walloc(st, st.bpart, size_t(h.nnzb) * 128, err))) return s;
     HIP_TRY(hipHostMalloc((void**)&st.S_host, size_t(st.ncol) * sizeof(double), hipHostMallocDefault));
     if ((s = walloc(st, st.x64, h.nnzb * NBLK, err))) return s;
     st.ready = true;
@@ -1180,8 +1197,12 @@ int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p, T* q, hipStr
     int const fp = sizeof(T) == 8 ? 64 : 32;
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
     dim3 const gb(st.nseg), tb(256), gs(st.nCols), ts(64 * FOLD_P);
-    if (int e = capi_action(plan, fp, q, p, s, err)) return e;
-    hipLaunchKernelGGL((k_cocg_pq<T>), gb, tb, 0, s, p, q, g);
+    // (p, A p) from the action's epilogue when nothing is added to A p after the local kernel (no numax 6, 7
+    // atoms), else from a pass over p and q
+    bool const fused = capi_dev(plan).n_wide_pairs == 0;
+    if (int e = capi_action(plan, fp, q, p, s, err, ACTION_ALL, fused ? st.bpart : nullptr)) return e;
+    if (fused) hipLaunchKernelGGL(k_fold_blocks, gb, dim3(64), 0, s, st.bpart, g);
+    else hipLaunchKernelGGL((k_cocg_pq<T>), gb, tb, 0, s, p, q, g);
     hipLaunchKernelGGL(k_cocg_alpha, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
     hipLaunchKernelGGL((k_cocg_r<T>), gb, tb, 0, s, r, q, st.S, st.ncol, g);
     hipLaunchKernelGGL(k_cocg_beta, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);

@@ -15,7 +15,7 @@ void solver_release(gfa_plan* plan);
 
 // provided by gfa_capi.cpp
 int capi_action(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, std::string& err,
-                int parts = ACTION_ALL);
+                int parts = ACTION_ALL, double* dot = nullptr);
 int capi_autotune(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, int reps, double* best_ms,
                   std::string& err);
 Plan const& capi_host(gfa_plan const* p);
