@@ -300,6 +300,7 @@ struct LocalArgs {
     real_t c8[3][5];         // 8th-order weights per direction
     real_t c16[3][9];        // 16th-order weights per direction
     int nh[3];               // neighbour blocks per side used in direction d (1 or 2)
+    real_t cdiag;            // sum of the centre weights of the three directions
     double phase[3][2][2];
     double hgrid[4];
 };
@@ -359,6 +360,19 @@ __device__ __forceinline__ void sho_add_atom(real_t* __restrict__ ys, real_t con
     }
 }
 
+// value of a neighbour block at point e (a compile-time element offset from the uniform pointer b) and byte
+// offset `off` (this thread's column and plane), with the Bloch phase of a periodic image applied:
+// f x_p - (+/-) g x_other; off_o is the byte offset of the other plane.  Uniform base + immediate + 32-bit
+// vector offset is the global load's own addressing mode: no per-load address arithmetic.
+template <typename real_t, bool IMG>
+__device__ __forceinline__ real_t nb_value(real_t const* __restrict__ b, int e, uint32_t off, uint32_t off_o,
+                                           real_t fr, real_t fi) {
+    char const* bb = reinterpret_cast<char const*>(b + e);
+    real_t v = *reinterpret_cast<real_t const*>(bb + off);
+    if (IMG) v = fr * v - fi * *reinterpret_cast<real_t const*>(bb + off_o);
+    return v;
+}
+
 // One work-group per (block, JW-column slice), 8*JW threads.  LDS holds the slice of the block
 // (xs) and the output accumulator (ys); passes: potential, kinetic x, y, z, SHO add-back, store.
 // In the kinetic passes a thread owns 4 lines (4 points each) of one (column, plane); neighbour
@@ -407,7 +421,7 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
         for (int k = 0; k < 2 * 64 * JW / T; ++k) {
             int const e = k * T + tid, jl = e % JW, row = e / JW, p = row >> 6, i = row & 63;
             real_t const Vim = E_im * real_t(1 - 2 * p);
-            ys[e] = (real_t(Vb[i]) * vm - E_re) * xs[e] + Vim * xs[((1 - p) * 64 + i) * JW + jl];
+            ys[e] = (real_t(Vb[i]) * vm + (a.cdiag - E_re)) * xs[e] + Vim * xs[((1 - p) * 64 + i) * JW + jl];
         }
     }
 
@@ -416,7 +430,8 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
     int const j = slice * JW + jl;
     real_t* ysp = ys + p * 64 * JW + jl;
     real_t const* xsp = xs + p * 64 * JW + jl;
-    size_t const off_p = size_t(p) * 4096 + j, off_o = size_t(1 - p) * 4096 + j;
+    // byte offsets of this thread's column in its plane and in the other plane (neighbour loads)
+    uint32_t const boff_p = uint32_t(p * 4096 + j) * sizeof(real_t), boff_o = uint32_t((1 - p) * 4096 + j) * sizeof(real_t);
 
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -448,19 +463,16 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
             else if (d == 1) i0 = (line >> 2) * 16 + (line & 3);     // line = z*4 + x
             else i0 = line;                                          // line = y*4 + x
             il0[r] = i0;
+            uint32_t const line_p = boff_p + uint32_t(i0) * (64 * sizeof(real_t));   // first point of the line
+            uint32_t const line_o = boff_o + uint32_t(i0) * (64 * sizeof(real_t));
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 int const i = i0 + q * stride;
                 w[r][4 * NH + q] = xsp[i * JW];
 #pragma unroll
                 for (int k = 0; k < NH; ++k) {
-                    real_t vl = nb[0][k][off_p + i * 64], vr = nb[1][k][off_p + i * 64];
-                    if (IMG) {
-                        vl = fr[0][k] * vl - fi[0][k] * nb[0][k][off_o + i * 64];
-                        vr = fr[1][k] * vr - fi[1][k] * nb[1][k][off_o + i * 64];
-                    }
-                    w[r][4 * (NH - 1 - k) + q] = vl;
-                    w[r][4 * (NH + 1 + k) + q] = vr;
+                    w[r][4 * (NH - 1 - k) + q] = nb_value<real_t, IMG>(nb[0][k], q * stride * 64, line_p, line_o, fr[0][k], fi[0][k]);
+                    w[r][4 * (NH + 1 + k) + q] = nb_value<real_t, IMG>(nb[1][k], q * stride * 64, line_p, line_o, fr[1][k], fi[1][k]);
                 }
             }
         }
@@ -470,17 +482,18 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
             for (int q = 0; q < 4; ++q) {
                 int const c = 4 * NH + q;
                 real_t t;
+                // off-centre weights only: the centre weights are applied with the potential
                 if (NH == 2 && a.nh[d] == 2) {
-                    t = a.c16[d][0] * w[r][c];
+                    t = a.c16[d][1] * (w[r][c - 1] + w[r][c + 1]);
 #pragma unroll
-                    for (int m = 1; m <= 8; ++m) {
+                    for (int m = 2; m <= 8; ++m) {
                         int const lo = c - m, hi = c + m;
                         t += a.c16[d][m] * (w[r][lo < 0 ? 0 : lo] + w[r][hi >= NW ? NW - 1 : hi]);
                     }
                 } else {
-                    t = a.c8[d][0] * w[r][c];
+                    t = a.c8[d][1] * (w[r][c - 1] + w[r][c + 1]);
 #pragma unroll
-                    for (int m = 1; m <= 4; ++m) t += a.c8[d][m] * (w[r][c - m] + w[r][c + m]);
+                    for (int m = 2; m <= 4; ++m) t += a.c8[d][m] * (w[r][c - m] + w[r][c + m]);
                 }
                 ysp[(il0[r] + q * stride) * JW] += t;
             }
@@ -606,32 +619,20 @@ __device__ __forceinline__ void sho_add_regs(real_t (&out)[16], real_t const (*H
     }
 }
 
-// value of a neighbour block at element offset `off` (this thread's plane) with the Bloch phase of a periodic
-// image applied: f x_p - (+/-) g x_other; off_o is the offset of the other plane
-template <typename real_t, bool IMG>
-__device__ __forceinline__ real_t nb_value(real_t const* __restrict__ b, size_t off, size_t off_o, real_t fr, real_t fi) {
-    real_t v = b[off];
-    if (IMG) v = fr * v - fi * b[off_o];
-    return v;
-}
-
-// the finite-difference weights applied to a window w of 4 (2 NH + 1) values centred at c
+// acc += the off-centre finite-difference weights applied to a window w of 4 (2 NH + 1) values centred at c
+// (the centre weights of the three directions are applied together with the potential)
 template <typename real_t, int NH, int NW>
-__device__ __forceinline__ real_t stencil(LocalArgs<real_t> const& a, int d, real_t const (&w)[NW], int c) {
-    real_t t;
+__device__ __forceinline__ void stencil(real_t& acc, LocalArgs<real_t> const& a, int d, real_t const (&w)[NW], int c) {
     if (NH == 2 && a.nh[d] == 2) {
-        t = a.c16[d][0] * w[c];
 #pragma unroll
         for (int m = 1; m <= 8; ++m) {
             int const lo = c - m, hi = c + m;
-            t += a.c16[d][m] * (w[lo < 0 ? 0 : lo] + w[hi >= NW ? NW - 1 : hi]);
+            acc += a.c16[d][m] * (w[lo < 0 ? 0 : lo] + w[hi >= NW ? NW - 1 : hi]);
         }
     } else {
-        t = a.c8[d][0] * w[c];
 #pragma unroll
-        for (int m = 1; m <= 4; ++m) t += a.c8[d][m] * (w[c - m] + w[c + m]);
+        for (int m = 1; m <= 4; ++m) acc += a.c8[d][m] * (w[c - m] + w[c + m]);
     }
-    return t;
 }
 
 // z-direction stencil for the thread's plane ZQ: window positions z' = ZQ - 4 NH .. ZQ + 4 NH along z at every
@@ -639,7 +640,7 @@ __device__ __forceinline__ real_t stencil(LocalArgs<real_t> const& a, int d, rea
 template <typename real_t, int NH, bool IMG, int JW, int ZQ>
 __device__ __forceinline__ void z_pass(LocalArgs<real_t> const& a, real_t (&acc)[16], real_t const (&own)[16],
                                        real_t const* __restrict__ xsp, real_t const* const (&nb)[2][NH],
-                                       real_t const (&fr)[2][NH], real_t const (&fi)[2][NH], size_t off_p, size_t off_o) {
+                                       real_t const (&fr)[2][NH], real_t const (&fi)[2][NH], uint32_t boff_p, uint32_t boff_o) {
     constexpr int NWZ = 8 * NH + 1;   // z' = ZQ - 4 NH .. ZQ + 4 NH
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -651,15 +652,13 @@ __device__ __forceinline__ void z_pass(LocalArgs<real_t> const& a, real_t (&acc)
             else if (z >= 0 && z < 4) w[m] = xsp[(z * 16 + k) * JW];
             else if (z < 0) {
                 int const kb = (-z - 1) / 4, zz = z + 4 * (kb + 1);
-                w[m] = nb_value<real_t, IMG>(nb[0][kb], off_p + size_t(zz * 16 + k) * 64, off_o + size_t(zz * 16 + k) * 64,
-                                             fr[0][kb], fi[0][kb]);
+                w[m] = nb_value<real_t, IMG>(nb[0][kb], (zz * 16 + k) * 64, boff_p, boff_o, fr[0][kb], fi[0][kb]);
             } else {
                 int const kb = (z - 4) / 4, zz = z - 4 * (kb + 1);
-                w[m] = nb_value<real_t, IMG>(nb[1][kb], off_p + size_t(zz * 16 + k) * 64, off_o + size_t(zz * 16 + k) * 64,
-                                             fr[1][kb], fi[1][kb]);
+                w[m] = nb_value<real_t, IMG>(nb[1][kb], (zz * 16 + k) * 64, boff_p, boff_o, fr[1][kb], fi[1][kb]);
             }
         }
-        acc[k] += stencil<real_t, NH, NWZ>(a, 2, w, 4 * NH);
+        stencil<real_t, NH, NWZ>(acc[k], a, 2, w, 4 * NH);
     }
 }
 
@@ -704,7 +703,8 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     int const j = slice * JW + jl;
     real_t const* xsp = xs + p * 64 * JW + jl;
     real_t const* xso = xs + (1 - p) * 64 * JW + jl;
-    size_t const off_p = size_t(p) * 4096 + j, off_o = size_t(1 - p) * 4096 + j;
+    // byte offsets of this thread's column in its plane and in the other plane (neighbour loads)
+    uint32_t const boff_p = uint32_t(p * 4096 + j) * sizeof(real_t), boff_o = uint32_t((1 - p) * 4096 + j) * sizeof(real_t);
 
     real_t own[16], acc[16];
 #pragma unroll
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
         real_t const Vim = E_im * real_t(1 - 2 * p);
 #pragma unroll
         for (int k = 0; k < 16; ++k)
-            acc[k] = (real_t(Vb[k]) * vm - E_re) * own[k] + Vim * xso[(zq * 16 + k) * JW];
+            acc[k] = (real_t(Vb[k]) * vm + (a.cdiag - E_re)) * own[k] + Vim * xso[(zq * 16 + k) * JW];
     }
 
 #pragma unroll
@@ -739,33 +739,40 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
                 fi[sd][k] = img ? real_t(a.phase[d][sd][1]) * real_t(1 - 2 * p) : real_t(0);
             }
         if (d < 2) {
-            // lines of this plane: along x (d = 0) the rows y, along y (d = 1) the columns x
+            // lines of this plane: along x (d = 0) the rows y, along y (d = 1) the columns x; zq is uniform
+            // within the wave, so the plane's offset goes into the (scalar) base pointers
+            int const zqu = __builtin_amdgcn_readfirstlane(zq);
+            real_t const* nbz[2][NH];
+#pragma unroll
+            for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+                for (int kk = 0; kk < NH; ++kk) nbz[sd][kk] = nb[sd][kk] + zqu * 16 * 64;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 real_t w[NW];   // the line through this block and NH neighbour blocks on each side
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);   // point of the plane on the line
-                    size_t const e = size_t(zq * 16 + k) * 64;
+                    int const e = k * 64;   // plane zq is in the base pointers
 #pragma unroll
                     for (int kk = 0; kk < NH; ++kk) {
-                        w[4 * (NH - 1 - kk) + q] = nb_value<real_t, IMG>(nb[0][kk], off_p + e, off_o + e, fr[0][kk], fi[0][kk]);
-                        w[4 * (NH + 1 + kk) + q] = nb_value<real_t, IMG>(nb[1][kk], off_p + e, off_o + e, fr[1][kk], fi[1][kk]);
+                        w[4 * (NH - 1 - kk) + q] = nb_value<real_t, IMG>(nbz[0][kk], e, boff_p, boff_o, fr[0][kk], fi[0][kk]);
+                        w[4 * (NH + 1 + kk) + q] = nb_value<real_t, IMG>(nbz[1][kk], e, boff_p, boff_o, fr[1][kk], fi[1][kk]);
                     }
                     w[4 * NH + q] = own[k];
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);
-                    acc[k] += stencil<real_t, NH, NW>(a, d, w, 4 * NH + q);
+                    stencil<real_t, NH, NW>(acc[k], a, d, w, 4 * NH + q);
                 }
             }
         } else {
             switch (zq) {
-                case 0: z_pass<real_t, NH, IMG, JW, 0>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
-                case 1: z_pass<real_t, NH, IMG, JW, 1>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
-                case 2: z_pass<real_t, NH, IMG, JW, 2>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
-                default: z_pass<real_t, NH, IMG, JW, 3>(a, acc, own, xsp, nb, fr, fi, off_p, off_o); break;
+                case 0: z_pass<real_t, NH, IMG, JW, 0>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
+                case 1: z_pass<real_t, NH, IMG, JW, 1>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
+                case 2: z_pass<real_t, NH, IMG, JW, 2>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
+                default: z_pass<real_t, NH, IMG, JW, 3>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
             }
         }
     }
@@ -804,18 +811,19 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
         }
     }
 
-    real_t* dst = a.y + blk * NBLK + off_p + size_t(zq * 16) * 64;
+    real_t* dst = a.y + blk * NBLK + size_t(p * 4096 + j) + size_t(zq * 16) * 64;
 #pragma unroll
     for (int k = 0; k < 16; ++k) dst[size_t(k) * 64] = acc[k];
     if (a.dot) {
         // (x, y) = sum_i x_i y_i per column: plane 0 holds x_r, y_r (+ x_i from LDS), plane 1 x_i, y_i (+ x_r):
         // Re = sum x_r y_r - sum x_i y_i, Im = sum x_i y_r + sum x_r y_i
-        double re = 0, im = 0;
+        real_t re_t = 0, im_t = 0;   // 16 products in the vector precision, then summed in FP64
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            re += double(own[k]) * double(acc[k]);
-            im += double(xo[k]) * double(acc[k]);
+            re_t += own[k] * acc[k];
+            im_t += xo[k] * acc[k];
         }
+        double re = re_t, im = im_t;
         if (p == 1) re = -re;
         __syncthreads();   // the LDS of the slice is free
         double* red = reinterpret_cast<double*>(xs);   // [zq][p][re, im][JW]
@@ -1280,13 +1288,16 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     a.dot = dot;
     static int const c8i[5] = {-14350, 8064, -1008, 128, -9};
     static int const c16i[9] = {-924708642, 538137600, -94174080, 22830080, -5350800, 1053696, -156800, 15360, -735};
+    double cdiag = 0;
     for (int dd = 0; dd < 3; ++dd) {
         double const n8 = s.prefactor[dd] / 5040., n16 = s.prefactor[dd] / 302702400.;
         for (int m = 0; m < 5; ++m) a.c8[dd][m] = real_t(c8i[m] * n8);
         for (int m = 0; m < 9; ++m) a.c16[dd][m] = real_t(c16i[m] * n16);
         a.nh[dd] = (d.fd_range[dd] == 8) ? 2 : 1;
+        cdiag += (a.nh[dd] == 2) ? c16i[0] * n16 : c8i[0] * n8;
         for (int u = 0; u < 2; ++u) for (int v = 0; v < 2; ++v) a.phase[dd][u][v] = s.phase[dd][u][v];
     }
+    a.cdiag = real_t(cdiag);
     for (int k = 0; k < 4; ++k) a.hgrid[k] = s.hgrid[k];
     // work-groups are dealt round-robin to the 8 XCDs: give each XCD a contiguous range of
     // (block, column slice) items so that neighbour blocks meet in the same L2

@@ -1,11 +1,13 @@
-"""The device tfQMR solve through the C-ABI.
+"""The device tfQMR and COCG solves through the C-ABI.
 
 Pins: (1) with the same seeded shadow residual the FP64 device solve follows the oracle's CPU TFQMR (same
 algorithm and scalar formulas), so the solutions agree far below the solve tolerance; (2) the true residual
 ||B - A X|| / ||B|| of every column of the device solution, measured with the oracle's action (itself equal to
 the reference's CPU action), meets the tolerance; (3) the graph-replayed and the eagerly launched solves are
 bit-identical (deterministic reductions); (4) the FP32 solve with FP64 refinement reaches the same tolerance
-and its solution agrees with the FP64 one within the conditioning of the problem."""
+and its solution agrees with the FP64 one within the conditioning of the problem; (5) COCG (complex-symmetric
+operators only), in FP64 and mixed, with every tile shape of the local kernels, meets the same residual bar and
+gives the tfQMR solution."""
 import numpy as np
 import pytest
 
@@ -131,3 +133,25 @@ def test_cocg_rejects_complex_phases(device):
     plan.set_inputs(E, k)
     with pytest.raises(G.GfaError, match="COCG"):
         plan.solve(fp_bits=64, tolerance=1e-9, max_iterations=100, method="cocg")
+
+
+@pytest.mark.parametrize("kind,columns,lines", [(0, 16, 1), (0, 16, 2), (0, 16, 4), (0, 32, 1), (0, 32, 2), (0, 32, 4),
+                                               (1, 32, 0), (1, 64, 0)])
+def test_cocg_with_every_tile_shape(device, kind, columns, lines):
+    """COCG takes (p, A p) from the local kernel's epilogue; with each tile shape the FP64 and the mixed solves
+    reach the tolerance (oracle-measured residual) and give the tfQMR solution."""
+    for name, mk, kw, E, k in [c for c in CASES if c[4] == (0.0, 0.0, 0.0)]:
+        plan = G.GreenPlan(mk(), **kw)
+        plan.set_inputs(E, k)
+        X64, _, _ = run(plan, device, fp_bits=64, tolerance=1e-9, max_iterations=2000, max_refinements=4)
+        ora = O.Oracle(plan, E, k)
+        for fp in (64, 32):
+            plan.set_tile_shape(fp, kind, columns, lines)
+            print(f"tile {kind}:{columns}x{lines} {name} cocg fp{fp}", flush=True)
+            Xc, _, sc = run(plan, device, fp_bits=fp, tolerance=1e-9, max_iterations=2000, max_refinements=6,
+                            method="cocg")
+            r = true_residuals(plan, ora, Xc)
+            err = np.abs(Xc - X64).max() / np.abs(X64).max()
+            assert sc["converged_columns"] == sc["total_columns"], (name, fp)
+            assert r.max() <= 1e-9, (name, fp, r.max())
+            assert err < 1e-6, (name, fp, err)
