@@ -305,59 +305,108 @@ struct LocalArgs {
     double hgrid[4];
 };
 
-// SHO add-back of one atom image into the LDS accumulator: this thread's z plane zq of column jl, plane p
-// dsrc points to coefficient 0 of this thread's (plane, column); consecutive coefficients are dstride apart.
-// The loops over SHO indices stay rolled: only the 4x4 spatial loops index register arrays, and rolled
-// loops keep the coefficient reads from being hoisted into registers all at once.
-template <typename real_t, int JW>
-__device__ __forceinline__ void sho_add_atom(real_t* __restrict__ ys, real_t const (*H)[3][4], float const (*xi2)[4],
-                                             float R2, real_t const* __restrict__ dsrc, size_t dstride, int zq, int L) {
-    int s = 0;
-    real_t acc[16];
+// one z layer iz of the contraction in sho_plane: the M = L - iz triangle of coefficients (iy + ix <= M, ix
+// fastest) times H_z[iz](zq), accumulated into c2[iy][ix] (triangular with rows of the full L triangle)
+template <typename real_t, int L, int M>
+__device__ __forceinline__ void sho_z_layer(real_t (&c2)[((L + 1) * (L + 2)) / 2], real_t const* __restrict__ src,
+                                            size_t dstride, real_t hz) {
+    int t = 0;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) acc[k] = 0;
+    for (int iy = 0; iy <= M; ++iy)
+#pragma unroll
+        for (int ix = 0; ix <= M - iy; ++ix, ++t) c2[(iy * (2 * L + 3 - iy)) / 2 + ix] += src[size_t(t) * dstride] * hz;
+}
+
+// sum over the SHO coefficients c[iz][iy][ix] (ix fastest, iz + iy + ix <= L) of one atom image at the 16
+// points (y, x) of the z plane zq, v(y, x) = sum c H_x[ix](x) H_y[iy](y) H_z[iz](zq), handed to out(y, x, v).
+// Contracted z first (one value H_z[iz](zq) per thread), then y and x row by row: (L+1)(L+2)(L+3)/6 +
+// 4 (L+1)(L+2)/2 + 16 (L+1) FMAs.  The z layers run in a rolled loop, so at most one layer of coefficients is
+// loaded ahead; ROLL keeps the row loop rolled as well (out must then accept a run-time row y).
+// dsrc points to coefficient 0 of this thread's (plane, column); consecutive coefficients are dstride apart.
+template <typename real_t, int L, bool ROLL, typename Out>
+__device__ __forceinline__ void sho_plane(real_t const (*H)[3][4], real_t const* __restrict__ dsrc, size_t dstride,
+                                          int zq, Out&& out) {
+    constexpr int n2 = ((L + 1) * (L + 2)) / 2;
+    real_t c2[n2];   // [iy][ix] triangular, row iy starts at iy (2L + 3 - iy) / 2
+#pragma unroll
+    for (int k = 0; k < n2; ++k) c2[k] = 0;
+    real_t const* src = dsrc;
 #pragma unroll 1
     for (int iz = 0; iz <= L; ++iz) {
-        real_t byx[4][4];
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-#pragma unroll
-            for (int xx = 0; xx < 4; ++xx) byx[y][xx] = 0;
-#pragma unroll 1
-        for (int iy = 0; iy <= L - iz; ++iy) {
-            real_t ax[4] = {0, 0, 0, 0};
-#pragma unroll 1
-            for (int ix = 0; ix <= L - iz - iy; ++ix) {
-                real_t const ca = dsrc[size_t(s) * dstride];
-                ++s;
-#pragma unroll
-                for (int xx = 0; xx < 4; ++xx) ax[xx] += H[ix][0][xx] * ca;
-            }
-#pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                real_t const hy_ = H[iy][1][y];
-#pragma unroll
-                for (int xx = 0; xx < 4; ++xx) byx[y][xx] += hy_ * ax[xx];
-            }
+        real_t const hz = H[iz][2][zq];
+        switch (L - iz) {
+            case 0: sho_z_layer<real_t, L, 0>(c2, src, dstride, hz); break;
+            case 1: if constexpr (L >= 1) sho_z_layer<real_t, L, 1>(c2, src, dstride, hz); break;
+            case 2: if constexpr (L >= 2) sho_z_layer<real_t, L, 2>(c2, src, dstride, hz); break;
+            case 3: if constexpr (L >= 3) sho_z_layer<real_t, L, 3>(c2, src, dstride, hz); break;
+            case 4: if constexpr (L >= 4) sho_z_layer<real_t, L, 4>(c2, src, dstride, hz); break;
+            case 5: if constexpr (L >= 5) sho_z_layer<real_t, L, 5>(c2, src, dstride, hz); break;
+            case 6: if constexpr (L >= 6) sho_z_layer<real_t, L, 6>(c2, src, dstride, hz); break;
+            default: if constexpr (L >= 7) sho_z_layer<real_t, L, 7>(c2, src, dstride, hz); break;
         }
-        real_t const hz_ = H[iz][2][zq];
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-#pragma unroll
-            for (int xx = 0; xx < 4; ++xx) acc[y * 4 + xx] += hz_ * byx[y][xx];
+        src += size_t(((L - iz + 1) * (L - iz + 2)) / 2) * dstride;
     }
-    float const d2z = xi2[2][zq] - R2;
+    auto row = [&](int y) {
+        real_t by[L + 1];
 #pragma unroll
-    for (int y = 0; y < 4; ++y) {
-        float const d2yz = xi2[1][y] + d2z;
+        for (int ix = 0; ix <= L; ++ix) by[ix] = 0;
+#pragma unroll
+        for (int iy = 0; iy <= L; ++iy) {
+            real_t const hy = H[iy][1][y];
+#pragma unroll
+            for (int ix = 0; ix <= L - iy; ++ix) by[ix] += c2[(iy * (2 * L + 3 - iy)) / 2 + ix] * hy;
+        }
 #pragma unroll
         for (int xx = 0; xx < 4; ++xx) {
-            float const d2 = xi2[0][xx] + d2yz;
-            bool const in = (d2z < 0) && (d2yz < 0) && (d2 < 0);
-            real_t& o = ys[(zq * 16 + y * 4 + xx) * JW];
-            o += in ? acc[y * 4 + xx] : real_t(0);
+            real_t v = 0;
+#pragma unroll
+            for (int ix = 0; ix <= L; ++ix) v += by[ix] * H[ix][0][xx];
+            out(y, xx, v);
         }
+    };
+    if constexpr (ROLL) {
+#pragma unroll 1
+        for (int y = 0; y < 4; ++y) row(y);
+    } else {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) row(y);
     }
+}
+
+// sho_plane for a run-time numax L <= LMAX (LMAX <= 7)
+template <typename real_t, int LMAX, bool ROLL, typename Out>
+__device__ __forceinline__ void sho_plane_any(real_t const (*H)[3][4], real_t const* __restrict__ dsrc, size_t dstride,
+                                              int zq, int L, Out&& out) {
+    switch (L) {
+        case 0: sho_plane<real_t, 0, ROLL>(H, dsrc, dstride, zq, out); break;
+        case 1: sho_plane<real_t, 1, ROLL>(H, dsrc, dstride, zq, out); break;
+        case 2: sho_plane<real_t, 2, ROLL>(H, dsrc, dstride, zq, out); break;
+        case 3: sho_plane<real_t, 3, ROLL>(H, dsrc, dstride, zq, out); break;
+        case 4: sho_plane<real_t, 4, ROLL>(H, dsrc, dstride, zq, out); break;
+        case 5: sho_plane<real_t, 5, ROLL>(H, dsrc, dstride, zq, out); break;
+        case 6: if constexpr (LMAX >= 6) sho_plane<real_t, 6, ROLL>(H, dsrc, dstride, zq, out); break;
+        default: if constexpr (LMAX >= 7) sho_plane<real_t, 7, ROLL>(H, dsrc, dstride, zq, out); break;
+    }
+}
+
+// is point (y, x) of z plane zq inside the projection sphere (the 3D mask evaluated in float from xi^2)
+__device__ __forceinline__ bool sho_inside(float const (*xi2)[4], float R2, int zq, int y, int xx) {
+    float const d2z = xi2[2][zq] - R2;
+    float const d2yz = xi2[1][y] + d2z;
+    float const d2 = xi2[0][xx] + d2yz;
+    return (d2z < 0) && (d2yz < 0) && (d2 < 0);
+}
+
+// SHO add-back of one atom image (numax L <= LMAX) into this thread's z plane zq of an accumulator in memory
+// (LDS or global) whose points are ostride apart: ys[(zq*16 + y*4 + x) * ostride]
+template <typename real_t, int LMAX>
+__device__ __forceinline__ void sho_add_atom(real_t* __restrict__ ys, int ostride, real_t const (*H)[3][4],
+                                             float const (*xi2)[4], float R2, real_t const* __restrict__ dsrc,
+                                             size_t dstride, int zq, int L) {
+    sho_plane_any<real_t, LMAX, true>(H, dsrc, dstride, zq, L, [&](int y, int xx, real_t v) {
+        real_t& o = ys[(zq * 16 + y * 4 + xx) * ostride];
+        o += sho_inside(xi2, R2, zq, y, xx) ? v : real_t(0);
+    });
 }
 
 // value of a neighbour block at point e (a compile-time element offset from the uniform pointer b) and byte
@@ -525,7 +574,7 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
                 }
             }
             __syncthreads();
-            sho_add_atom<real_t, JW>(ysp, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
+            sho_add_atom<real_t, 5>(ysp, JW, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
         }
     }
     __syncthreads();
@@ -567,56 +616,13 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
 }
 
 
-// SHO add-back of one atom image into 16 register accumulators (z plane zq of one column and plane); the same
-// sum-factorised contraction as sho_add_atom.
+// SHO add-back of one atom image (numax <= 5) into 16 register accumulators (z plane zq of one column and plane)
 template <typename real_t>
 __device__ __forceinline__ void sho_add_regs(real_t (&out)[16], real_t const (*H)[3][4], float const (*xi2)[4],
                                              float R2, real_t const* __restrict__ dsrc, size_t dstride, int zq, int L) {
-    int s = 0;
-    real_t acc[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) acc[k] = 0;
-#pragma unroll 1
-    for (int iz = 0; iz <= L; ++iz) {
-        real_t byx[4][4];
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-#pragma unroll
-            for (int xx = 0; xx < 4; ++xx) byx[y][xx] = 0;
-#pragma unroll 1
-        for (int iy = 0; iy <= L - iz; ++iy) {
-            real_t ax[4] = {0, 0, 0, 0};
-#pragma unroll 1
-            for (int ix = 0; ix <= L - iz - iy; ++ix) {
-                real_t const ca = dsrc[size_t(s) * dstride];
-                ++s;
-#pragma unroll
-                for (int xx = 0; xx < 4; ++xx) ax[xx] += H[ix][0][xx] * ca;
-            }
-#pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                real_t const hy_ = H[iy][1][y];
-#pragma unroll
-                for (int xx = 0; xx < 4; ++xx) byx[y][xx] += hy_ * ax[xx];
-            }
-        }
-        real_t const hz_ = H[iz][2][zq];
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-#pragma unroll
-            for (int xx = 0; xx < 4; ++xx) acc[y * 4 + xx] += hz_ * byx[y][xx];
-    }
-    float const d2z = xi2[2][zq] - R2;
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-        float const d2yz = xi2[1][y] + d2z;
-#pragma unroll
-        for (int xx = 0; xx < 4; ++xx) {
-            float const d2 = xi2[0][xx] + d2yz;
-            bool const in = (d2z < 0) && (d2yz < 0) && (d2 < 0);
-            out[y * 4 + xx] += in ? acc[y * 4 + xx] : real_t(0);
-        }
-    }
+    sho_plane_any<real_t, 5, false>(H, dsrc, dstride, zq, L, [&](int y, int xx, real_t v) {
+        out[y * 4 + xx] += sho_inside(xi2, R2, zq, y, xx) ? v : real_t(0);
+    });
 }
 
 // acc += the off-centre finite-difference weights applied to a window w of 4 (2 NH + 1) values centred at c
@@ -863,7 +869,7 @@ __global__ __launch_bounds__(512) void k_sho_add_wide(LocalArgs<real_t> a, uint3
     real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + a.colindx[blk]) * 128 + p * 64 + j;
     real_t* yb = a.y + size_t(blk) * NBLK + p * 4096 + j;
     // ys stride 64 for the direct update of y
-    sho_add_atom<real_t, 64>(yb, H, xi2, R2, dsrc, size_t(a.nCols) * 128, zq, L);
+    sho_add_atom<real_t, 7>(yb, 64, H, xi2, R2, dsrc, size_t(a.nCols) * 128, zq, L);
 }
 
 template <typename T>
