@@ -379,6 +379,19 @@ int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int l
     });
 }
 
+int gfa_get_tile_shape(gfa_plan* plan, int fp_bits, int* kind, int* columns, int* lines, int* tuned) {
+    return guarded([&]() -> int {
+        if (!plan || !kind || !columns || !lines || !tuned) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        int const v = fp_bits == 64 ? 0 : 1;
+        *kind = plan->dev.tune_kind[v];
+        *columns = plan->dev.tune_jw[v];
+        *lines = plan->dev.tune_lb[v];
+        *tuned = plan->dev.tuned[v] ? 1 : 0;
+        return GFA_OK;
+    });
+}
+
 int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, int reps, double* best_ms) {
     return guarded([&]() -> int {
         if (!plan || !y || !x) return fail(GFA_ERR_ARGUMENT, "null argument");

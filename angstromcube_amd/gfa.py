@@ -24,7 +24,7 @@ EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS",
 EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
-             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_action_part", "gfa_gradient",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_action_part", "gfa_gradient",
              "gfa_solve", "gfa_last_error"]
 
 
@@ -92,6 +92,8 @@ def lib():
                                           C.POINTER(C.c_double)]
         L.gfa_autotune.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
         L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        ip = C.POINTER(C.c_int)
+        L.gfa_get_tile_shape.argtypes = [vp, C.c_int, ip, ip, ip, ip]
         L.gfa_action_part.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gfa_gradient.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), vp]
@@ -300,6 +302,12 @@ class GreenPlan:
 
     def set_tile_shape(self, fp_bits, kind, columns, lines=0):
         _check(lib().gfa_set_tile_shape(self.h, fp_bits, kind, columns, lines))
+
+    def tile_shape(self, fp_bits):
+        """(kind, columns, lines, tuned) of the local kernel that the action of fp_bits launches."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().gfa_get_tile_shape(self.h, fp_bits, *[C.byref(t) for t in v]))
+        return tuple(t.value for t in v)
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
               shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0, method="tfqmr"):

@@ -25,6 +25,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BLOCK_BYTES = {64: 2 * 64 * 64 * 8, 32: 2 * 64 * 64 * 4}
+# FP32 local-kernel tile shapes (kind, columns, lines) timed by --mode action
+FP32_SHAPES = [(0, 16, 1), (0, 16, 2), (0, 16, 4), (0, 32, 1), (0, 32, 2), (0, 32, 4), (1, 32, 0), (1, 64, 0)]
+# HBM bytes per launch of the FP32 local kernel per tile shape on this workload (rocprofv3 FETCH_SIZE and
+# WRITE_SIZE of --mode action, scaled by a plain copy's counts); the command that made them is in the file
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "local_fp32_contour32_traffic.json")
 SOURCE = (16, 16, 16)
 
 
@@ -108,10 +113,15 @@ def main():
 
     if args.mode == "action":
         p = batch_plan(G, sysm, batch(instances, 0, n))
-        for fp in (64, 32):
-            for part in (3, 2):
-                t = time_action(p, fp, part, args.action_reps, stream)
-                log(f"# action fp{fp} part {part}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+        for part in (3, 2):
+            t = time_action(p, 64, part, args.action_reps, stream)
+            log(f"# action fp64 part {part}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+        t = time_action(p, 32, 3, args.action_reps, stream)
+        log(f"# action fp32 part 3: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+        for shape in FP32_SHAPES:   # each FP32 local-kernel shape, in this order (per-dispatch counters follow it)
+            p.set_tile_shape(32, *shape)
+            t = time_action(p, 32, 2, args.action_reps, stream)
+            log(f"# local fp32 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
         # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
         x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
         y = torch.empty_like(x)
@@ -197,6 +207,7 @@ def main():
     # ---- roofline of the local kernel (potential + kinetic + SHO add-back), HIP events on its stream ----
     t_loc64 = time_action(plan, 64, 2, args.action_reps, stream)
     t_loc32 = time_action(plan, 32, 2, args.action_reps, stream)
+    shape32 = plan.tile_shape(32)[:3]   # the FP32 local kernel the mixed solve ran (autotuned)
     t_act64 = time_action(plan, 64, 3, args.action_reps, stream)
     t_act32 = time_action(plan, 32, 3, args.action_reps, stream)
     nrows = plan.nRows
@@ -232,13 +243,29 @@ def main():
         "action": {"fp64_ms": round(t_act64 * 1e3, 3), "fp32_ms": round(t_act32 * 1e3, 3),
                    "local_fp64_ms": round(t_loc64 * 1e3, 3), "local_fp32_ms": round(t_loc32 * 1e3, 3),
                    "fp64_GBs": round(alg64 / t_act64 / 1e9, 1), "fp32_GBs": round(alg32 / t_act32 / 1e9, 1)},
-        "roofline": {"bound": "hbm", "achieved": round(alg64 / t_loc64 / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(alg64 / t_loc64 / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_local FP64 (potential + 8th-order kinetic + SHO add-back); algorithmic bytes "
-                               "per launch = nnzb * 2 * 64 KiB + nRows * 512 B"},
+        "roofline": {"bound": "hbm", "achieved": round(alg32 / t_loc32 / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg32 / t_loc32 / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_of(shape32, nnzb),
+                     "kernel": f"FP32 local kernel, tile shape {shape32} ({'k_local' if shape32[0] == 0 else 'k_plane'}: "
+                               "potential + 8th-order kinetic + SHO add-back), the one the mixed solve ran; "
+                               "algorithmic bytes per launch = nnzb * 2 * 32 KiB + nRows * 512 B"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def traffic_of(shape, nnzb):
+    """HBM bytes (read + written) per launch of the FP32 local kernel of this tile shape, from the committed
+    rocprofv3 counter summary of the same workload; None if it covers another workload or shape."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("shapes", {}).get(",".join(str(v) for v in shape))
+    if t.get("nnzb") != int(nnzb) or e is None:
+        return None
+    return int(e["fetch_bytes"] + e["write_bytes"])
 
 
 def cpu_baseline(sysm, first, half_steps_mixed, half_steps_fp64, args):

@@ -151,6 +151,9 @@ int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stre
 /* pins the tile shape instead: kind 0 (LDS accumulator): columns per work-group 16 or 32, kinetic lines per
  * thread in flight 1, 2 or 4; kind 1 (register accumulator per z plane): columns 32 or 64, lines ignored */
 int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int lines);
+/* the tile shape the action of fp_bits uses (the autotuner's choice, the pinned one or the default); *tuned = 1
+ * when it was set by gfa_autotune or gfa_set_tile_shape */
+int gfa_get_tile_shape(gfa_plan* plan, int fp_bits, int* kind, int* columns, int* lines, int* tuned);
 
 /* ---- the solve (H - E S) X = B with B = unit columns on the diagonal blocks; replaces
  *      green_solver_t::solve / action_t::solve (include/green_solver.hxx:24-30, include/green_action.hxx:181-241)

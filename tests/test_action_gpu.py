@@ -120,8 +120,11 @@ def test_autotune_keeps_the_action_exact(device):
     x = _rand((plan.nnzb, 2, 64, 64), 22)
     xd = torch.from_numpy(x).to(device)
     yd = torch.empty_like(xd)
+    assert plan.tile_shape(64)[3] == 0   # not tuned yet
     best = plan.autotune(yd, xd, fp_bits=64, reps=2)
     assert best > 0
+    kind, columns, lines, tuned = plan.tile_shape(64)
+    assert tuned == 1 and (kind, columns) in [(0, 16), (0, 32), (1, 32), (1, 64)]
     plan.multiply(yd, xd, fp_bits=64)
     torch.cuda.synchronize()
     y_ref = O.Oracle(plan, E, k).action(x)
@@ -130,6 +133,8 @@ def test_autotune_keeps_the_action_exact(device):
         plan.set_tile_shape(64, 0, 24, 2)
     with pytest.raises(G.GfaError, match="tile"):
         plan.set_tile_shape(64, 1, 16)
+    plan.set_tile_shape(32, 1, 64)
+    assert plan.tile_shape(32) == (1, 64, 0, 1)
 
 
 def test_action_parts_compose(device):
