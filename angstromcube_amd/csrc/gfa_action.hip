@@ -136,6 +136,11 @@ __global__ __launch_bounds__(128) void k_sho_project(
         for (int z = 0; z < 4; ++z) {
             float const d2z = xi2[2][z] - R2;
             if (d2z < 0) {
+                // the 16 points of the plane are loaded together (unconditionally, all in the block) and the
+                // sphere mask is applied as a select, so the loads are in flight at once
+                real_t v[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) v[k] = xb[64 * (z * 16 + k)];
                 real_t byx[n2];
 #pragma unroll
                 for (int k = 0; k < n2; ++k) byx[k] = 0;
@@ -149,11 +154,9 @@ __global__ __launch_bounds__(128) void k_sho_project(
 #pragma unroll
                         for (int xx = 0; xx < 4; ++xx) {
                             float const d2 = xi2[0][xx] + d2yz;
-                            if (d2 < 0) {
-                                real_t const ps = xb[64 * ((z * 4 + y) * 4 + xx)];
+                            real_t const ps = (d2 < 0) ? v[y * 4 + xx] : real_t(0);
 #pragma unroll
-                                for (int k = 0; k <= L; ++k) ax[k] += ps * H[k][0][xx];
-                            }
+                            for (int k = 0; k <= L; ++k) ax[k] += ps * H[k][0][xx];
                         }
 #pragma unroll
                         for (int iy = 0; iy <= L; ++iy) {
