@@ -205,23 +205,15 @@ __global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
     extern __shared__ double lds[];
     uint32_t const a = blockIdx.x, c = blockIdx.y;
     if (col_active && !col_active[c]) return;  // a retired column group
-    int const j = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // the wave index is uniform: the rows i = w, w + ATOM_WAVES, ... of M are read with scalar loads
+    int const j = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int const L = atom_lmax[a];
     int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
     bool const in_lds = nc <= 64;
     double* cre = in_lds ? lds : cscratch + (size_t(atom_starts[a]) * nCols + size_t(c) * nc) * 128;
     double* cim = cre + nc * 64;
-    // M = (h - E_c s) dVol of this column's energy, staged in LDS (after the coefficients) for broadcast reads
-    double* Mre = in_lds ? lds + 2 * nc * 64 : nullptr;
-    double* Mim = in_lds ? Mre + nc * nc : nullptr;
     double const* HS = atom_mat + atom_mat_off[a];  // [2][nc][nc]: h dVol, s dVol
     double const Er = E_col[2 * c], Ei = E_col[2 * c + 1];
-    if (in_lds)
-        for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
-            double const hd = HS[e], sd = HS[nc * nc + e];
-            Mre[e] = hd - Er * sd;
-            Mim[e] = -Ei * sd;
-        }
     uint32_t const k0 = sum_start[a], k1 = sum_start[a + 1];
     for (int s = w; s < nc; s += ATOM_WAVES) {
         double ar = 0, ai = 0;
@@ -238,25 +230,27 @@ __global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
             ar += ph_re * vr - ph_im * vi;
             ai += ph_re * vi + ph_im * vr;
         }
-        cre[s * 64 + j] = ar;
-        cim[s * 64 + j] = ai;
+        if (in_lds) { lds[s * 64 + j] = ar; lds[(nc + s) * 64 + j] = ai; }   // LDS addressed as such
+        else { cre[s * 64 + j] = ar; cim[s * 64 + j] = ai; }
     }
     __syncthreads();
     // d = M c with M = (h - E s) dVol of this column's energy, complex double accumulation in ascending
-    // column order (include/green_dyadic.hxx:800-819, src/green_function.cxx:97-98)
-    for (int i = w; i < nc; i += ATOM_WAVES) {
-        double dr = 0, di = 0;
+    // column order (include/green_dyadic.hxx:800-819, src/green_function.cxx:97-98); the rows of M are
+    // wave-uniform (scalar loads), the coefficients per column come from LDS (numax <= 5) or the scratch
+    auto mv = [&](int i, double const* __restrict__ cr_, double const* __restrict__ ci_, double& dr, double& di) {
+#pragma unroll 4
         for (int k = 0; k < nc; ++k) {
-            double mr, mi;
-            if (in_lds) { mr = Mre[i * nc + k]; mi = Mim[i * nc + k]; }
-            else {
-                double const hd = HS[i * nc + k], sd = HS[nc * nc + i * nc + k];
-                mr = hd - Er * sd; mi = -Ei * sd;
-            }
-            double const cr = cre[k * 64 + j], ci = cim[k * 64 + j];
+            double const hd = HS[i * nc + k], sd = HS[nc * nc + i * nc + k];
+            double const mr = hd - Er * sd, mi = -Ei * sd;
+            double const cr = cr_[k * 64 + j], ci = ci_[k * 64 + j];
             dr += mr * cr - mi * ci;
             di += mr * ci + mi * cr;
         }
+    };
+    for (int i = w; i < nc; i += ATOM_WAVES) {
+        double dr = 0, di = 0;
+        if (in_lds) mv(i, lds, lds + nc * 64, dr, di);
+        else mv(i, cre, cim, dr, di);
         for (uint32_t k = k0; k < k1; ++k) {
             uint32_t const img = sum_index[k];
             double const ph_re = image_phase[2 * img], ph_im = -image_phase[2 * img + 1];
@@ -1276,7 +1270,7 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
             HIP_TRY(hipGetLastError());
         }
         uint32_t const ncl = std::min<uint32_t>(d.max_atom_nc, 64);   // larger atoms use the global scratch
-        size_t const lds = (size_t(2) * ncl * 64 + size_t(2) * ncl * ncl) * sizeof(double);
+        size_t const lds = size_t(2) * ncl * 64 * sizeof(double);
         hipLaunchKernelGGL((k_sho_atoms<real_t>), dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), lds, stream, ws.dcoef,
                            ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.image_starts,
                            d.atom_lmax, d.atom_hs, d.atom_mat_off, d.image_phase, d.nCols, d.cscratch,
