@@ -84,6 +84,7 @@ int capi_autotune(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t 
     return device_autotune<float>(p->dev, s, (float*)y, (float const*)x, stream, reps, best_ms, err);
 }
 Plan const& capi_host(gfa_plan const* p) { return p->host; }
+void capi_set_error(std::string const& msg) { g_err = msg; }
 DevicePlan& capi_dev(gfa_plan* p) { return p->dev; }
 }  // namespace gfa
 

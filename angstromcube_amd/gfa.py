@@ -25,7 +25,8 @@ EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
              "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_action_part", "gfa_gradient",
-             "gfa_solve", "gfa_last_error"]
+             "gfa_solve", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density",
+             "gfa_last_error"]
 
 
 class GfaError(RuntimeError):
@@ -55,6 +56,17 @@ class SolveOpts(C.Structure):
                 ("echo", C.c_int32), ("eager", C.c_int32), ("autotune", C.c_int32),
                 ("fp32_max_iterations", C.c_int32), ("residuals", C.POINTER(C.c_double)), ("halfsteps", C.POINTER(C.c_int32)),
                 ("method", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ContourOpts(C.Structure):
+    _fields_ = [("E", C.POINTER(C.c_double)), ("wE", C.POINTER(C.c_double)), ("nE", C.c_int32), ("nk", C.c_int32),
+                ("k", C.POINTER(C.c_double)), ("E_Fermi", C.c_double), ("dVol", C.c_double),
+                ("max_groups", C.c_int32), ("reserved", C.c_int32), ("solve", SolveOpts)]
+
+
+class ContourStats(C.Structure):
+    _fields_ = [("solves", C.c_int32), ("sources", C.c_int32), ("converged_columns", C.c_int32),
+                ("total_columns", C.c_int32), ("max_residual", C.c_double), ("seconds", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -98,6 +110,12 @@ def lib():
         L.gfa_gradient.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), vp]
         L.gfa_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_double), C.POINTER(SolveStats)]
+        dp = C.POINTER(C.c_double)
+        L.gfa_gauss_legendre.argtypes = [C.c_int32, dp, dp]
+        L.gfa_gauss_fermi_dirac.argtypes = [C.c_int32, dp, dp]
+        L.gfa_energy_mesh.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp,
+                                      C.c_int32, C.POINTER(C.c_int32)]
+        L.gfa_contour_density.argtypes = [vp, dp, dp, C.POINTER(ContourOpts), dp, dp, C.POINTER(ContourStats)]
         L.gfa_last_error.restype = C.c_char_p
         for name in C_SYMBOLS:
             getattr(L, name).restype = C.c_int if name != "gfa_last_error" else C.c_char_p
@@ -112,6 +130,40 @@ def _check(status):
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def plan_desc(system, r_trunc, fd_range=(4, 4, 4), nsources=(1, 1, 1), sources=None, r_proj=6.0, echo=0,
+              source_radii=None):
+    """The gfa_plan_desc of a system; returns (desc, arrays the desc points into)."""
+    d = PlanDesc()
+    keep = []
+    for k in range(3):
+        d.ng[k] = system.ng[k]
+        d.bc[k] = system.bc[k]
+        d.hg[k] = system.h
+        d.fd_range[k] = fd_range[k]
+        d.nsources[k] = nsources[k]
+    xyz = np.ascontiguousarray(system.xyzZinso, dtype=np.float64)
+    keep.append(xyz)
+    d.xyzZinso = _dptr(xyz)
+    d.natoms = system.natoms
+    d.echo = echo
+    d.r_trunc = r_trunc
+    d.r_proj = r_proj
+    d.scale_T = 1.0
+    src = None
+    if sources is not None:
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32).reshape(-1, 3))
+        keep.append(src)
+        d.sources = src.ctypes.data_as(C.POINTER(C.c_int32))
+        d.nsources_explicit = src.shape[0]
+    if source_radii is not None:
+        radii = np.ascontiguousarray(source_radii, dtype=np.float64)
+        if src is None or radii.shape != (src.shape[0],):
+            raise GfaError("one truncation radius per explicit source block is needed")
+        keep.append(radii)
+        d.source_radii = _dptr(radii)
+    return d, keep
 
 
 def nsho_of(numax):
@@ -129,29 +181,7 @@ class GreenPlan:
     def __init__(self, system, r_trunc, fd_range=(4, 4, 4), nsources=(1, 1, 1), sources=None, r_proj=6.0,
                  echo=0, source_radii=None):
         L = lib()
-        d = PlanDesc()
-        for k in range(3):
-            d.ng[k] = system.ng[k]
-            d.bc[k] = system.bc[k]
-            d.hg[k] = system.h
-            d.fd_range[k] = fd_range[k]
-            d.nsources[k] = nsources[k]
-        self._xyz = np.ascontiguousarray(system.xyzZinso, dtype=np.float64)
-        d.xyzZinso = _dptr(self._xyz)
-        d.natoms = system.natoms
-        d.echo = echo
-        d.r_trunc = r_trunc
-        d.r_proj = r_proj
-        d.scale_T = 1.0
-        if sources is not None:
-            self._src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32).reshape(-1, 3))
-            d.sources = self._src.ctypes.data_as(C.POINTER(C.c_int32))
-            d.nsources_explicit = self._src.shape[0]
-        if source_radii is not None:
-            self._radii = np.ascontiguousarray(source_radii, dtype=np.float64)
-            if sources is None or self._radii.shape != (self._src.shape[0],):
-                raise GfaError("one truncation radius per explicit source block is needed")
-            d.source_radii = _dptr(self._radii)
+        d, self._keep = plan_desc(system, r_trunc, fd_range, nsources, sources, r_proj, echo, source_radii)
         h = C.c_void_p()
         _check(L.gfa_plan_create(C.byref(d), C.byref(h)))
         self.h = h

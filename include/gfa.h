@@ -211,6 +211,52 @@ int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve
 int gfa_gradient(gfa_plan* plan, const void* X, const void* Lambda, const double* E, const double* w, double* grad_V,
                  double* grad_hs, void* stream);
 
+/* ---- the energy contour around the solve (src/energy_contour.cxx) ----
+ * gfa_gauss_legendre replaces Gauss_Legendre_quadrature (src/energy_contour.cxx:456): n points on [-1, 1] in
+ *   ascending order, n rounded up as the reference's tables are (above 32 to a multiple of 4, above 64 to a
+ *   multiple of 8); returns the number of points written (0 for n <= 0), or minus an error status beyond 112.
+ * gfa_gauss_fermi_dirac replaces Gauss_Fermi_Dirac_quadrature (src/energy_contour.cxx:2156): the Gauss rule of
+ *   the weight 1 / (1 + exp(30 x)) on [-1, infinity), n <= 16.
+ * gfa_energy_mesh replaces get_energy_mesh (src/energy_contour.cxx:139-241): the contour points relative to the
+ *   Fermi level and their complex weights, E[nE][2] and w[nE][2]; with E or w NULL only *nE is returned. */
+int gfa_gauss_legendre(int32_t n, double* x, double* w);
+int gfa_gauss_fermi_dirac(int32_t n, double* x, double* w);
+int gfa_energy_mesh(double kBT, double eBot, int32_t nBot, int32_t nPar, int32_t nFer, int32_t nPol, double* E,
+                    double* w, int32_t capacity, int32_t* nE);
+
+typedef struct {
+    const double* E;            /* [nE][2] energy points relative to the Fermi level (gfa_energy_mesh)          */
+    const double* wE;           /* [nE][2] complex weights                                                     */
+    int32_t nE;
+    int32_t nk;                 /* k-points                                                                    */
+    const double* k;            /* [nk][4]: k (units of 2 pi) and its weight (brillouin_zone::WEIGHT)          */
+    double E_Fermi;             /* added to every energy point                                                 */
+    double dVol;                /* grid volume element (0: hx hy hz)                                           */
+    int32_t max_groups;         /* column groups per batched solve (0: 64); energies are batched per solve     */
+    int32_t reserved;
+    gfa_solve_opts solve;       /* options of every batched solve (fp_bits, tolerance, method, ...)            */
+} gfa_contour_opts;
+
+typedef struct {
+    int32_t solves;             /* batched solves run (energy batches x k-points)                              */
+    int32_t sources;            /* source blocks of the description                                            */
+    int32_t converged_columns;  /* columns that met the tolerance, over all solves                             */
+    int32_t total_columns;
+    double max_residual;
+    double seconds;
+} gfa_contour_stats;
+
+/* ---- contour integration: replaces energy_contour::Integrator::integrate (src/energy_contour.cxx:273-451) up to
+ *      the 4x4x4 density.  For the sources of desc (one column group each, one truncation radius), the potential
+ *      V[ng_z][ng_y][ng_x] and the raw atom matrices hs (per atom [2][nc][nc] in atom order, as
+ *      gfa_set_atom_matrices takes them), it solves every (energy, k-point, source) instance, energies batched as
+ *      column groups of one plan, and returns rho[nsources][64] = Re sum_E w_E sum_k w_k G_ii / pi and, if not NULL,
+ *      the response density rho_response[nsources][64] of the reference (-sum over all but the last two energy
+ *      points, divided by the same sum of their energies).  The 8x8x8 interpolation and the Fermi-level update
+ *      stay with the caller, as in the reference. ---- */
+int gfa_contour_density(const gfa_plan_desc* desc, const double* V, const double* hs, const gfa_contour_opts* opts,
+                        double* rho, double* rho_response, gfa_contour_stats* stats);
+
 /* text of the last error of the calling thread ("" if none) */
 const char* gfa_last_error(void);
 

@@ -8,6 +8,8 @@
 //   - green_function::update_energy_parameter    (src/green_function.cxx:64-109)
 //   - green_function::update_phases              (src/green_function.cxx:218-251)
 //   - green_action::action_t<real_t,2,1>::multiply (include/green_action.hxx:140-176)
+//   - energy_contour::Gauss_Legendre_quadrature, ::Gauss_Fermi_Dirac_quadrature, ::get_energy_mesh
+//     (src/energy_contour.cxx:139-253, :456, :2156)
 // Inputs are injected directly into the plan instead of the MPI exchange (serial case):
 //   - the local potential is stored cell-indexed, which is how the kernels read it
 //     (include/green_potential.hxx:90-91 with veff_index from src/green_function.cxx:913-916);
@@ -30,6 +32,12 @@
 #include "sho_tools.hxx"
 #include "sho_projection.hxx"
 #include "boundary_condition.hxx"
+#include "energy_contour.hxx"
+
+namespace energy_contour {   // defined in src/energy_contour.cxx without a header declaration
+std::vector<std::complex<double>> get_energy_mesh(std::vector<std::complex<double>>& w8, double kBT, double eBot,
+                                                  unsigned nBot, unsigned nPar, unsigned nFer, int nPol, int echo);
+}
 
 namespace {
 
@@ -270,6 +278,23 @@ int ref_selftest(int which, int echo) {
         case 4: return int(green_function::all_tests(echo));
     }
     return -1;
+}
+
+// the reference's quadratures: number of points (or the reference's error return) and the tables
+int ref_gauss_legendre(unsigned n, double* x, double* w) { return energy_contour::Gauss_Legendre_quadrature(x, w, n, 0); }
+int ref_gauss_fermi_dirac(unsigned n, double* x, double* w) { return energy_contour::Gauss_Fermi_Dirac_quadrature(x, w, n, 0); }
+
+// the reference's energy mesh: E[n][2], w[n][2] (capacity entries), returns the number of points
+int ref_energy_mesh(double kBT, double eBot, unsigned nBot, unsigned nPar, unsigned nFer, int nPol, double* E,
+                    double* w, int capacity) {
+    std::vector<std::complex<double>> w8;
+    auto const Ep = energy_contour::get_energy_mesh(w8, kBT, eBot, nBot, nPar, nFer, nPol, 0);
+    int const n = int(Ep.size());
+    for (int i = 0; i < n && i < capacity; ++i) {
+        E[2 * i] = Ep[i].real(); E[2 * i + 1] = Ep[i].imag();
+        w[2 * i] = w8[i].real(); w[2 * i + 1] = w8[i].imag();
+    }
+    return n;
 }
 
 } // extern "C"
