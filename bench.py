@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-actions", type=int, default=4)
-    ap.add_argument("--mode", choices=["bench", "action"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "action", "contour"], default="bench",
                     help="action: only apply the action a few times (for profiler runs)")
     return ap.parse_args()
 
@@ -129,6 +129,10 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         log(f"# copy calibration: {x.numel() * 8} bytes read and written per copy")
+        return
+
+    if args.mode == "contour":
+        contour_mode(G, sysm, args)
         return
 
     inner = args.inner_tol if args.inner_tol > 0 else (1e-30 if args.method == "cocg" else 1e-4)
@@ -251,6 +255,41 @@ def main():
                                "algorithmic bytes per launch = nnzb * 2 * 32 KiB + nRows * 512 B"},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out))
+
+
+def contour_mode(G, sysm, args):
+    """A whole contour integration (gfa_contour_density): the reference's default 32-point energy mesh, Gamma, 8
+    source blocks (2x2x2 at the cell centre), radius 8 Bohr.  Batched mixed precision (8 energies x 8 sources
+    per plan) against the reference's structure (one FP64 tfQMR solve of the 8 sources per energy point); prints
+    one JSON line with both rates and the densities' agreement."""
+    import numpy as np
+    import torch
+    from angstromcube_amd import contour as K
+    E, wE = K.energy_mesh()
+    kw = dict(nsources=(2, 2, 2), kpoints=[(0.0, 0.0, 0.0, 1.0)], E_Fermi=0.0, tolerance=1e-9)
+    runs = {}
+    for name, opts in (("batched_mixed_cocg", dict(max_groups=64, fp_bits=32, method="cocg", max_refinements=6,
+                                                    inner_tolerance=1e-30, fp32_max_iterations=3000,
+                                                    max_iterations=8000, autotune=True)),
+                       ("per_energy_fp64_tfqmr", dict(max_groups=8, fp_bits=64, method="tfqmr", max_refinements=2,
+                                                      max_iterations=8000))):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rho, res, st = K.contour_density(sysm, 8.0, E, wE, **kw, **opts)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        runs[name] = (rho, st, dt)
+        log(f"# contour {name}: {st['converged_columns']}/{st['total_columns']} columns in {dt:.2f} s, "
+            f"{st['solves']} solves, max residual {st['max_residual']:.2e}")
+    (r1, s1, t1), (r2, s2, t2) = runs["batched_mixed_cocg"], runs["per_energy_fp64_tfqmr"]
+    out = {"mode": "contour", "energy_points": int(len(E)), "kpoints": 1, "sources": int(s1["sources"]),
+           "radius_bohr": 8.0, "columns": int(s1["total_columns"]),
+           "batched_mixed_columns_per_s": round(s1["converged_columns"] / t1, 2), "batched_mixed_seconds": round(t1, 3),
+           "per_energy_fp64_columns_per_s": round(s2["converged_columns"] / t2, 2),
+           "per_energy_fp64_seconds": round(t2, 3), "speedup": round(t2 / t1, 3),
+           "density_max_rel_diff": float(np.abs(r1 - r2).max() / np.abs(r2).max()),
+           "electrons_in_source_blocks": float(r2.sum() * sysm.dvol)}
     print(json.dumps(out))
 
 
