@@ -1373,6 +1373,50 @@ template int device_autotune<float>(DevicePlan&, ActionScalars const&, float*, f
 // column group; wE[c] = {w_c, -E_c w_c}.  Results on the host: gcell_host[ncell][64] (dL/dV per cell point) and
 // gmat_host at atom_mat_off (per contributing atom [2][nc][nc], the derivative with respect to the entries of
 // (h dVol-scaled, s dVol-scaled) before the SHO prefactors, see gfa_gradient).
+// projection of one FP64 vector onto every atom image and collection per (atom, column group) with the Bloch
+// phases: out[((atom_starts[a] + s) * nCols + c)][re/im][j]
+static int project_collect(DevicePlan& d, ActionScalars const& s, double const* vec, double* out, hipStream_t stream,
+                           std::string& err) {
+    auto& ws = d.ws64;
+    if (!ws.partial) {
+        if (int r = dalloc(d, ws.partial, size_t(d.total_chunk_coefs) * 128, err)) return r;
+        if (int r = dalloc(d, ws.dcoef, size_t(d.natomcoeffs) * d.nCols * 128, err)) return r;
+    }
+    for (int L = 0; L <= 7; ++L) {
+        uint32_t const n = d.bucket_start[L + 1] - d.bucket_start[L];
+        if (!n) continue;
+        uint32_t const* bucket = d.bucket_chunks + d.bucket_start[L];
+#define LAUNCH_PRJ(LL) hipLaunchKernelGGL((k_sho_project<double, LL>), dim3(n), dim3(128), 0, stream, ws.partial, vec, \
+        d.chunks, bucket, d.chunk_coef, d.prj_blocks, d.rowindx, d.row_pos, d.image_pos, \
+        s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], (uint8_t const*)nullptr, d.htab, d.xtab, d.prj_slot)
+        switch (L) {
+            case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
+            case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;
+            case 4: LAUNCH_PRJ(4); break; case 5: LAUNCH_PRJ(5); break;
+            case 6: LAUNCH_PRJ(6); break; default: LAUNCH_PRJ(7); break;
+        }
+#undef LAUNCH_PRJ
+    }
+    hipLaunchKernelGGL(k_sho_collect, dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), 0, stream, out, ws.partial,
+                       d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.atom_lmax, d.image_phase, d.nCols,
+                       d.atom_starts);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int device_atom_coefficients(DevicePlan& d, ActionScalars const& s, double const* X, std::vector<double>& host,
+                             hipStream_t stream, std::string& err) {
+    size_t const n = size_t(d.ncoeffs) * d.nCols * 128;
+    host.assign(n, 0.0);
+    if (d.nAtoms == 0 || d.nchunks == 0 || n == 0) return 0;
+    if (!d.atom_coef)   // kept with the plan, like the other work buffers
+        if (int r = dalloc(d, d.atom_coef, n, err)) return r;
+    if (int r = project_collect(d, s, X, d.atom_coef, stream, err)) return r;
+    HIP_TRY(hipMemcpyAsync(host.data(), d.atom_coef, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+}
+
 int device_gradient(DevicePlan& d, Plan const& p, ActionScalars const& s, double const* X, double const* Lam,
                     std::vector<double> const& wE, std::vector<double>& gcell_host, std::vector<double>& gmat_host,
                     hipStream_t stream, std::string& err) {
@@ -1404,33 +1448,8 @@ int device_gradient(DevicePlan& d, Plan const& p, ActionScalars const& s, double
                        g.cell_blocks);
     HIP_TRY(hipGetLastError());
     if (d.nAtoms > 0 && d.nchunks > 0) {
-        auto& ws = d.ws64;
-        if (!ws.partial) {
-            if (int r = dalloc(d, ws.partial, size_t(d.total_chunk_coefs) * 128, err)) return r;
-            if (int r = dalloc(d, ws.dcoef, size_t(d.natomcoeffs) * d.nCols * 128, err)) return r;
-        }
-        for (int v = 0; v < 2; ++v) {
-            double const* vec = v ? Lam : X;
-            for (int L = 0; L <= 7; ++L) {
-                uint32_t const n = d.bucket_start[L + 1] - d.bucket_start[L];
-                if (!n) continue;
-                uint32_t const* bucket = d.bucket_chunks + d.bucket_start[L];
-#define LAUNCH_PRJ(LL) hipLaunchKernelGGL((k_sho_project<double, LL>), dim3(n), dim3(128), 0, stream, ws.partial, vec, \
-                d.chunks, bucket, d.chunk_coef, d.prj_blocks, d.rowindx, d.row_pos, d.image_pos, \
-                s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], (uint8_t const*)nullptr, d.htab, d.xtab, d.prj_slot)
-                switch (L) {
-                    case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
-                    case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;
-                    case 4: LAUNCH_PRJ(4); break; case 5: LAUNCH_PRJ(5); break;
-                    case 6: LAUNCH_PRJ(6); break; default: LAUNCH_PRJ(7); break;
-                }
-#undef LAUNCH_PRJ
-            }
-            hipLaunchKernelGGL(k_sho_collect, dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), 0, stream, v ? g.cL : g.cX,
-                               ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.atom_lmax,
-                               d.image_phase, d.nCols, d.atom_starts);
-            HIP_TRY(hipGetLastError());
-        }
+        for (int v = 0; v < 2; ++v)
+            if (int r = project_collect(d, s, v ? Lam : X, v ? g.cL : g.cX, stream, err)) return r;
         hipLaunchKernelGGL(k_grad_atoms, dim3(d.nAtoms, d.max_atom_nc), dim3(64), 0, stream, g.gmat, g.cX, g.cL,
                            d.atom_lmax, d.atom_starts, d.atom_mat_off, g.wE, d.nCols);
         HIP_TRY(hipGetLastError());

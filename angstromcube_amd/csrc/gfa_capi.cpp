@@ -2,7 +2,11 @@
 // failures through a status code and gfa_last_error(); none terminates the process.
 #include "../../include/gfa.h"
 
+#include <algorithm>
+#include <cmath>
+#include <complex>
 #include <cstring>
+#include <vector>
 #include <new>
 #include <string>
 
@@ -300,6 +304,88 @@ int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* 
         std::string err;
         if (int s = gfa::capi_action(plan, fp_bits, y, x, (hipStream_t)stream, err, part))
             return fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
+        return GFA_OK;
+    });
+}
+
+int gfa_atom_density_matrices(gfa_plan* plan, const void* X, double factor, double* D, void* stream) {
+    return guarded([&]() -> int {
+        if (!plan || !X || !D) return fail(GFA_ERR_ARGUMENT, "null argument");
+        gfa::Plan const& h = plan->host;
+        if (h.nAtomImages != h.nAtoms)
+            return fail(GFA_ERR_ARGUMENT, "atomic density matrices need one image per atom: the reduction over periodic "
+                                          "images is not defined (include/green_dyadic.hxx:1085 asserts the same)");
+        if (int s = ensure_device(plan)) return s;
+        std::vector<double> cx;
+        std::string err;
+        if (gfa::device_atom_coefficients(plan->dev, scalars(h), (double const*)X, cx, (hipStream_t)stream, err))
+            return fail(GFA_ERR_DEVICE, err);
+        std::vector<size_t> off(h.nAtoms + 1, 0);
+        for (uint32_t a = 0; a < h.nAtoms; ++a) off[a + 1] = off[a] + size_t(gfa::nsho(h.atom_lmax[a])) * gfa::nsho(h.atom_lmax[a]);
+        std::fill(D, D + off[h.nAtoms], 0.0);
+        typedef std::complex<double> cplx;
+        float const R2 = float(h.r_proj * h.r_proj);
+        for (uint32_t iai = 0; iai < h.nAtomImages; ++iai) {   // SHOprj_right (include/green_dyadic.hxx:925-1036)
+            int const L = h.image_lmax[iai];
+            int const nc = gfa::nsho(L), n2 = ((L + 1) * (L + 2)) / 2;
+            uint32_t const a = h.image_atom[iai], a0 = h.atom_starts[a];
+            double const* apos = &h.image_pos[4 * size_t(iai)];
+            double const sig_inv = apos[3] * apos[3];
+            // Hermite functions of this image at the 4x4x4 points of every source block: H[c][nu][dir][i4], the
+            // 1D functions of hermite_1d (include/green_dyadic.hxx:33-107) evaluated on the host
+            std::vector<double> H(size_t(h.nCols) * (L + 1) * 12);
+            std::vector<float> x2(size_t(h.nCols) * 12);
+            for (uint32_t c = 0; c < h.nCols; ++c)
+                for (int dir = 0; dir < 3; ++dir)
+                    for (int i4 = 0; i4 < 4; ++i4) {
+                        double const xi = sig_inv * ((double(h.col_pos[4 * c + dir]) * 4 + i4 + 0.5) * h.hg[dir] - apos[dir]);
+                        double const xx = xi * xi;
+                        x2[c * 12 + dir * 4 + i4] = float(xx);
+                        double hn = (xx < h.r_proj * h.r_proj) ? std::exp(-0.5 * xx) : 0.0, hm = 0;
+                        H[(c * (L + 1) + 0) * 12 + dir * 4 + i4] = hn;
+                        for (int nu = 0; nu < L; ++nu) {
+                            double const hp = xi * hn - (0.5 * nu) * hm;
+                            H[(c * (L + 1) + nu + 1) * 12 + dir * 4 + i4] = hp;
+                            hm = hn; hn = hp;
+                        }
+                    }
+            std::vector<cplx> piGp(size_t(nc) * nc, 0.0), czyx(nc), byx(n2), ax(L + 1);
+            for (int isho = 0; isho < nc; ++isho) {
+                std::fill(czyx.begin(), czyx.end(), cplx(0));
+                for (uint32_t c = 0; c < h.nCols; ++c) {
+                    double const* src = &cx[((size_t(a0) + isho) * h.nCols + c) * 128];
+                    auto Hc = [&](int nu, int dir, int i4) { return H[(c * (L + 1) + nu) * 12 + dir * 4 + i4]; };
+                    for (int z4 = 0; z4 < 4; ++z4) {
+                        float const d2z = x2[c * 12 + 8 + z4] - R2;
+                        if (!(d2z < 0)) continue;
+                        std::fill(byx.begin(), byx.end(), cplx(0));
+                        for (int y4 = 0; y4 < 4; ++y4) {
+                            float const d2yz = x2[c * 12 + 4 + y4] + d2z;
+                            if (!(d2yz < 0)) continue;
+                            std::fill(ax.begin(), ax.end(), cplx(0));
+                            for (int x4 = 0; x4 < 4; ++x4) {
+                                if (!(x2[c * 12 + x4] + d2yz < 0)) continue;
+                                int const xyz = (z4 * 4 + y4) * 4 + x4;
+                                cplx const pG(src[xyz], src[64 + xyz]);
+                                for (int ix = 0; ix <= L; ++ix) ax[ix] += pG * Hc(ix, 0, x4);
+                            }
+                            for (int iy = 0; iy <= L; ++iy) {
+                                int const iyx0 = (iy * (2 * L + 3 - iy)) >> 1;
+                                for (int ix = 0; ix <= L - iy; ++ix) byx[iyx0 + ix] += ax[ix] * Hc(iy, 1, y4);
+                            }
+                        }
+                        for (int jsho = 0, iz = 0; iz <= L; ++iz)
+                            for (int iy = 0; iy <= L - iz; ++iy) {
+                                int const iyx0 = (iy * (2 * L + 3 - iy)) >> 1;
+                                for (int ix = 0; ix <= L - iz - iy; ++ix) czyx[jsho++] += byx[iyx0 + ix] * Hc(iz, 2, z4);
+                            }
+                    }
+                }
+                for (int jsho = 0; jsho < nc; ++jsho) piGp[size_t(isho) * nc + jsho] = czyx[jsho];
+            }
+            cplx const ph(h.image_phase[2 * iai], h.image_phase[2 * iai + 1]);
+            for (size_t ij = 0; ij < piGp.size(); ++ij) D[off[a] + ij] += (piGp[ij] * ph).imag() * factor;
+        }
         return GFA_OK;
     });
 }

@@ -104,6 +104,7 @@ struct DevicePlan {
 
     Workspace<double> ws64;
     GradWs grad;
+    double* atom_coef = nullptr;       // [ncoeffs][nCols][2][64] atom projections (density matrices)
     Workspace<float> ws32;
     size_t bytes = 0;
     std::vector<void*> allocations;
@@ -132,6 +133,10 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
 template <typename real_t>
 int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream, int reps,
                     double* best_ms, std::string& err);
+// the projections of an FP64 vector X onto every atom, collected over its images with the Bloch phases, on the
+// host: [((atom_starts[a] + s) * nCols + c)][re/im][j]
+int device_atom_coefficients(DevicePlan& d, ActionScalars const& s, double const* X, std::vector<double>& host,
+                             hipStream_t stream, std::string& err);
 int device_gradient(DevicePlan& d, Plan const& p, ActionScalars const& s, double const* X, double const* Lam,
                     std::vector<double> const& wE, std::vector<double>& gcell_host, std::vector<double>& gmat_host,
                     hipStream_t stream, std::string& err);

@@ -25,7 +25,7 @@ EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
              "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_action_part", "gfa_gradient",
-             "gfa_solve", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density",
+             "gfa_solve", "gfa_atom_density_matrices", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density",
              "gfa_last_error"]
 
 
@@ -111,6 +111,7 @@ def lib():
                                    C.POINTER(C.c_double), vp]
         L.gfa_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_double), C.POINTER(SolveStats)]
         dp = C.POINTER(C.c_double)
+        L.gfa_atom_density_matrices.argtypes = [vp, vp, C.c_double, dp, vp]
         L.gfa_gauss_legendre.argtypes = [C.c_int32, dp, dp]
         L.gfa_gauss_fermi_dirac.argtypes = [C.c_int32, dp, dp]
         L.gfa_energy_mesh.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp,
@@ -305,6 +306,21 @@ class GreenPlan:
         import torch
         st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         _check(lib().gfa_action_part(self.h, fp_bits, part, C.c_void_p(y.data_ptr()), C.c_void_p(x.data_ptr()), st))
+
+    def atom_density_matrices(self, X, factor=1.0, stream=None):
+        """Per contributing atom (ATOM_IDS order) the matrix factor * Im(phase <p_i|X|p_j>) of an FP64 device
+        tensor X in the plan layout, the right projection over every source block of the plan."""
+        import torch
+        lm = self.export("ATOM_LMAX").astype(int)
+        ncs = [nsho_of(l) for l in lm]
+        D = np.zeros(sum(n * n for n in ncs))
+        st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+        _check(lib().gfa_atom_density_matrices(self.h, C.c_void_p(X.data_ptr()), float(factor), _dptr(D), st))
+        out, o = [], 0
+        for n in ncs:
+            out.append(D[o:o + n * n].reshape(n, n))
+            o += n * n
+        return out
 
     def gradient(self, X, Lam, E, w, stream=None):
         """dL/dV (grid array) and dL/d(h, s) per atom (list of (2, nc, nc)) of L = Im sum_c w_c Tr[G_c], from the

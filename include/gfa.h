@@ -211,6 +211,14 @@ int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve
 int gfa_gradient(gfa_plan* plan, const void* X, const void* Lambda, const double* E, const double* w, double* grad_V,
                  double* grad_hs, void* stream);
 
+/* ---- atomic density matrices <p_i|G|p_j>: replaces green_dyadic::get_projection_coefficients with SHOprj_right
+ *      (include/green_dyadic.hxx:925-1091).  X: an FP64 Green function in the plan layout (device).  D receives, per
+ *      contributing atom of the plan (gfa_plan_export ATOM_IDS order), the nc x nc matrix
+ *      factor * Im(phase * sum_columns <p_i|G|s> p_j(s)), where the right projection runs over the points s of every
+ *      source block of the plan; concatenated over atoms.  As in the reference, plans whose atoms have periodic
+ *      images are rejected. ---- */
+int gfa_atom_density_matrices(gfa_plan* plan, const void* X, double factor, double* D, void* stream);
+
 /* ---- the energy contour around the solve (src/energy_contour.cxx) ----
  * gfa_gauss_legendre replaces Gauss_Legendre_quadrature (src/energy_contour.cxx:456): n points on [-1, 1] in
  *   ascending order, n rounded up as the reference's tables are (above 32 to a multiple of 4, above 64 to a

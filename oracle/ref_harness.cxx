@@ -8,6 +8,7 @@
 //   - green_function::update_energy_parameter    (src/green_function.cxx:64-109)
 //   - green_function::update_phases              (src/green_function.cxx:218-251)
 //   - green_action::action_t<real_t,2,1>::multiply (include/green_action.hxx:140-176)
+//   - green_dyadic::get_projection_coefficients  (include/green_dyadic.hxx:925-1091)
 //   - energy_contour::Gauss_Legendre_quadrature, ::Gauss_Fermi_Dirac_quadrature, ::get_energy_mesh
 //     (src/energy_contour.cxx:139-253, :456, :2156)
 // Inputs are injected directly into the plan instead of the MPI exchange (serial case):
@@ -295,6 +296,18 @@ int ref_energy_mesh(double kBT, double eBot, unsigned nBot, unsigned nPar, unsig
         w[2 * i] = w8[i].real(); w[2 * i + 1] = w8[i].imag();
     }
     return n;
+}
+
+// the reference's atomic density matrices of a Green function G[nnzb][2][64][64] (host memory):
+// green_dyadic::get_projection_coefficients (include/green_dyadic.hxx:1040-1091); concatenated per atom into out,
+// returns the number of values
+int ref_density_matrices(void* h, double const* G, double* out) {
+    auto& p = ((RefPlan*)h)->p;
+    auto const D = green_dyadic::get_projection_coefficients<double,2,1>((double const (*)[2][64][64])G, p.dyadic_plan,
+                                                                         p.rowindx, p.rowCubePos, p.colCubePos, 0);
+    size_t o = 0;
+    for (auto const& m : D) { std::memcpy(out + o, m.data(), m.size()*sizeof(double)); o += m.size(); }
+    return int(o);
 }
 
 } // extern "C"

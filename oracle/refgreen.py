@@ -41,6 +41,7 @@ def lib():
         L.ref_action32.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.ref_action_part64.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p]
         L.ref_selftest.argtypes = [C.c_int, C.c_int]
+        L.ref_density_matrices.argtypes = [vp, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -127,6 +128,14 @@ class RefPlan:
         fn = lib().ref_action64 if fp == 64 else lib().ref_action32
         fn(self.h, _p(y), _p(x))
         return y
+
+    def density_matrices(self, G):
+        """The reference's atomic density matrices of G (nnzb, 2, 64, 64), concatenated per atom."""
+        G = np.ascontiguousarray(G, dtype=np.float64)
+        assert G.shape == (self.nnzb, 2, 64, 64)
+        out = np.zeros(64 * 64 * 128)
+        n = lib().ref_density_matrices(self.h, _p(G), _p(out))
+        return out[:n].copy()
 
     def action_part(self, which, x, y=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
