@@ -138,10 +138,11 @@ def plan_desc(system, r_trunc, fd_range=(4, 4, 4), nsources=(1, 1, 1), sources=N
     """The gfa_plan_desc of a system; returns (desc, arrays the desc points into)."""
     d = PlanDesc()
     keep = []
+    hg = getattr(system, "hg", None) or (system.h,) * 3   # per-direction spacings where the system has them
     for k in range(3):
         d.ng[k] = system.ng[k]
         d.bc[k] = system.bc[k]
-        d.hg[k] = system.h
+        d.hg[k] = hg[k]
         d.fd_range[k] = fd_range[k]
         d.nsources[k] = nsources[k]
     xyz = np.ascontiguousarray(system.xyzZinso, dtype=np.float64)

@@ -82,7 +82,7 @@ class Oracle:
         op = OraclePlan()
         for d in range(3):
             op.nb[d] = sysm.ng[d] // 4
-            op.hg[d] = sysm.h
+            op.hg[d] = (getattr(sysm, "hg", None) or (sysm.h,) * 3)[d]
             op.nlists[d] = s.nlists[d]
             op.kin_start[d] = keep["KIN_START_" + "XYZ"[d]].ctypes.data
             op.kin_list[d] = keep["KIN_LIST_" + "XYZ"[d]].ctypes.data

@@ -57,7 +57,7 @@ class RefPlan:
         L = lib()
         ng = (C.c_uint32 * 3)(*system.ng)
         bc = (C.c_int8 * 3)(*system.bc)
-        hg = (C.c_double * 3)(system.h, system.h, system.h)
+        hg = (C.c_double * 3)(*(getattr(system, "hg", None) or (system.h,) * 3))
         self._xyz = np.ascontiguousarray(system.xyzZinso, dtype=np.float64)
         fd = (C.c_int32 * 3)(*fd_range)
         ns = (C.c_int32 * 3)(*nsrc)
