@@ -31,11 +31,11 @@ def flatten_hs(hs):
     return np.concatenate([np.asarray(m, dtype=np.float64).reshape(-1) for m in hs])
 
 
-def solve_fp64(plan, E, tol, maxit):
+def solve_fp64(plan, E, tol, maxit, method="tfqmr"):
     """X of (H - E_c S) X = B for every column group (FP64 device tensor) and the solve statistics."""
     plan.update_energies(E)
     X = torch.empty((plan.nnzb, 2, 64, 64), dtype=torch.float64, device="cuda")
-    rho, st = plan.solve(fp_bits=64, tolerance=tol, max_iterations=maxit, max_refinements=4, X=X)
+    rho, st = plan.solve(fp_bits=64, tolerance=tol, max_iterations=maxit, max_refinements=4, X=X, method=method)
     if st["converged_columns"] != st["total_columns"]:
         raise G.GfaError(f"solve did not converge: {st['converged_columns']}/{st['total_columns']} columns, "
                          f"max residual {st['max_residual']:.2e}")
@@ -50,26 +50,29 @@ def trace_value(rho, w):
 class ContourTrace(torch.autograd.Function):
     """L = Im sum_c w_c Tr[G_c] as a function of the grid potential V (ng_z, ng_y, ng_x) and the flattened atom
     matrices (h, s per atom, as given to gfa_set_atom_matrices).  Arguments after those two are constants:
-    the plan (GreenPlan, one column group per energy), energies E[nCols], weights w[nCols], tolerance, iterations."""
+    the plan (GreenPlan, one column group per energy), energies E[nCols], weights w[nCols], tolerance, iterations
+    and the Krylov method ("cocg" needs real Bloch phases: Gamma or a wrapped cell; the adjoint system then is
+    complex symmetric too)."""
 
     @staticmethod
-    def forward(ctx, V, hs_flat, plan, E, w, tol=1e-11, maxit=4000):
+    def forward(ctx, V, hs_flat, plan, E, w, tol=1e-11, maxit=4000, method="tfqmr"):
         sysm = plan.system
         plan.update_potential(V.detach().cpu().numpy(), _split_hs(sysm, hs_flat.detach().cpu().numpy()))
-        X, rho = solve_fp64(plan, E, tol, maxit)
+        X, rho = solve_fp64(plan, E, tol, maxit, method)
         ctx.plan, ctx.E, ctx.w, ctx.tol, ctx.maxit = plan, np.asarray(E, dtype=complex), np.asarray(w, dtype=complex), tol, maxit
+        ctx.method = method
         ctx.X = X
         return torch.tensor(trace_value(rho, w), dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, gL):
         plan, E, w = ctx.plan, ctx.E, ctx.w
-        Lam, _ = solve_fp64(plan, np.conj(E), ctx.tol, ctx.maxit)   # (H - conj(E) S) Lambda = B
+        Lam, _ = solve_fp64(plan, np.conj(E), ctx.tol, ctx.maxit, ctx.method)   # (H - conj(E) S) Lambda = B
         plan.update_energies(E)
         gV, ghs = plan.gradient(ctx.X, Lam, E, w)
         g = float(gL)
-        return (torch.from_numpy(gV * g), torch.from_numpy(flatten_hs(ghs) * g), None, None, None, None, None)
+        return (torch.from_numpy(gV * g), torch.from_numpy(flatten_hs(ghs) * g), None, None, None, None, None, None)
 
 
-def contour_trace(V, hs_flat, plan, E, w, tol=1e-11, maxit=4000):
-    return ContourTrace.apply(V, hs_flat, plan, E, w, tol, maxit)
+def contour_trace(V, hs_flat, plan, E, w, tol=1e-11, maxit=4000, method="tfqmr"):
+    return ContourTrace.apply(V, hs_flat, plan, E, w, tol, maxit, method)

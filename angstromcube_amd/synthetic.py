@@ -157,6 +157,12 @@ def contour_32(seed=0xA43C0003, natoms=128, ngauss=64):
     return make_system(seed, (32, 32, 32), ngauss=ngauss, natoms=natoms, numax=5)
 
 
+def contour_16(seed=0xA43C0004, natoms=16, ngauss=16):
+    """16x16x16 blocks (64^3 grid), periodic (wrapped by a 6 Bohr truncation), 16 atoms numax 3: the system of the
+    adjoint-gradient checks."""
+    return make_system(seed, (16, 16, 16), ngauss=ngauss, natoms=natoms, numax=3)
+
+
 def contour_instances(n_energies=32, n_kpoints=4, radii=(6.0, 8.0, 10.0, 12.0), seed=0xA43C0003):
     """(E, k, radius) instances of a batched contour integration.
 

@@ -201,3 +201,61 @@ def _unit(V, p):
     u = torch.zeros_like(V)
     u[p] = 1.0
     return u
+
+
+@pytest.mark.gpu
+def test_autograd_on_the_16_cube_contour(device):
+    """The adjoint gradient on the 16^3-block cell (64^3 grid, 16 atoms numax 3), 8 contour points on one source
+    block (radius 6 Bohr, wrapped), against central differences of GPU forward solves (COCG to 1e-14) at 32 seeded
+    grid points around the source and 8 entries of the contributing atoms' matrices (the four largest gradient
+    components and four seeded ones).
+
+    Two steps per direction.  At step 1e-6 Ha the quotient carries the roundoff noise of the FP64 forward solves
+    of L (~5e-17 absolute here, L ~ 1e-2; tighter solve tolerances do not lower it) divided by the step: ~6e-11,
+    which is 1e-6 of the largest gradient component; the bound there is 2e-6 of it.  L is smooth in V and in the
+    matrices, so at step 1e-4 Ha (8 of the grid points, all matrix entries) the noise is 100 times smaller (~1e-8 of the largest component) and the
+    truncation error (step^2) still below it: there the bound is 3e-8 of the largest component."""
+    import torch
+    from angstromcube_amd import adjoint as A
+    sysm = S.contour_16()
+    x, wl = np.polynomial.legendre.leggauss(8)
+    phi = 0.5 * np.pi * (x + 1.0)
+    E = -0.5 - 0.5 * np.cos(phi) + 1j * (0.05 + 0.25 * np.sin(phi))
+    w = wl * (0.5 * np.sin(phi) + 1j * 0.25 * np.cos(phi)) * 0.5 * np.pi
+    plan = G.GreenPlan(sysm, r_trunc=6.0, sources=[(8, 8, 8)] * 8)
+    assert plan.nCols == 8
+    V = torch.from_numpy(sysm.V.copy()).requires_grad_(True)
+    hs = torch.from_numpy(A.flatten_hs(sysm.hs)).requires_grad_(True)
+    L = A.contour_trace(V, hs, plan, E, w, tol=1e-14, method="cocg")
+    gV, ghs = torch.autograd.grad(L, (V, hs))
+    print(f"16^3 contour: nnzb {plan.nnzb}, L = {float(L.detach()):.10e}, max |dL/dV| {float(gV.abs().max()):.3e}, "
+          f"max |dL/dhs| {float(ghs.abs().max()):.3e}", flush=True)
+    rng = np.random.default_rng(16)
+    centre = 8 * 4 + 2   # grid index of the source block's centre along each axis
+    pts = [tuple(int(centre + rng.integers(-10, 11)) for _ in range(3)) for _ in range(32)]
+    ids = set(plan.export("ATOM_IDS").astype(int).tolist())
+    offs = np.cumsum([0] + [2 * m.shape[-1] ** 2 for m in sysm.hs])
+    ents = [int(e) for e in np.argsort(-ghs.abs().numpy())[:4]]
+    while len(ents) < 8:
+        ia = int(rng.integers(sysm.natoms))
+        if ia in ids:
+            ents.append(int(offs[ia] + rng.integers(offs[ia + 1] - offs[ia])))
+    # step 1e-4 on the first 8 grid points and on every matrix entry (keeps the test near a minute)
+    cases = [("V", p, gV, lambda d: (V + d, hs), i < 8) for i, p in enumerate(pts)] + \
+        [("hs", e, ghs, lambda d: (V, hs + d), True) for e in ents]
+    worst = {1e-6: 0.0, 1e-4: 0.0}
+    with torch.no_grad():
+        for name, idx, g, args, both in cases:
+            u = torch.zeros_like(g)
+            u[idx] = 1.0
+            fds = {1e-4: float("nan")}
+            for step in ((1e-6, 1e-4) if both else (1e-6,)):
+                Lp, Lm = [float(A.contour_trace(*args(sgn * step * u), plan, E, w, tol=1e-14, method="cocg"))
+                          for sgn in (+1, -1)]
+                fds[step] = (Lp - Lm) / (2 * step)
+                worst[step] = max(worst[step], abs(fds[step] - float(g[idx])) / float(g.abs().max()))
+            print(f"{name}[{idx}]: adjoint {float(g[idx]):.10e} fd(1e-6) {fds[1e-6]:.10e} fd(1e-4) {fds[1e-4]:.10e}",
+                  flush=True)
+            assert abs(fds[1e-6] - float(g[idx])) <= 2e-6 * float(g.abs().max())
+            assert not both or abs(fds[1e-4] - float(g[idx])) <= 3e-8 * float(g.abs().max())
+    print(f"largest deviation / largest component: step 1e-6 {worst[1e-6]:.2e}, step 1e-4 {worst[1e-4]:.2e}")
