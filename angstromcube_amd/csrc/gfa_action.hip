@@ -665,6 +665,147 @@ __device__ __forceinline__ void z_pass(LocalArgs<real_t> const& a, real_t (&acc)
     }
 }
 
+// neighbour blocks of blk in direction d, NH per side: base pointers (an absent neighbour reads the zero block)
+// and the Bloch phase of a periodic image as (fr, fi) for plane p (fr = 1, fi = 0 otherwise)
+template <typename real_t, int NH, bool IMG>
+__device__ __forceinline__ void neighbours(LocalArgs<real_t> const& a, int d, uint64_t blk, int p,
+                                           real_t const* (&nb)[2][NH], real_t (&fr)[2][NH], real_t (&fi)[2][NH]) {
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+            uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
+            nb[sd][k] = (code == -1) ? a.zero : a.x + idx * NBLK;
+            bool const img = IMG && (code <= -2);
+            fr[sd][k] = img ? real_t(a.phase[d][sd][0]) : real_t(1);
+            fi[sd][k] = img ? real_t(a.phase[d][sd][1]) * real_t(1 - 2 * p) : real_t(0);
+        }
+}
+
+// (V - E) x at the thread's 16 points (z plane zq of one column, plane p): own = x of plane p there, xso = x of
+// the other plane in LDS (row stride JW), the imaginary part of E coupling the two planes; the centre weights of
+// the three finite-difference directions are applied here as well
+template <typename real_t, int JW>
+__device__ __forceinline__ void plane_potential(LocalArgs<real_t> const& a, uint64_t blk, int p, int zq,
+                                                real_t const (&own)[16], real_t const* xso, real_t (&acc)[16]) {
+    int32_t const iv = a.veff[blk];
+    real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
+    double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + zq * 16;
+    uint32_t const colE = a.colindx[blk];
+    real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
+    real_t const Vim = E_im * real_t(1 - 2 * p);
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        acc[k] = (real_t(Vb[k]) * vm + (a.cdiag - E_re)) * own[k] + Vim * xso[(zq * 16 + k) * JW];
+}
+
+// kinetic x and y at the thread's 16 points: the lines of plane zq along x (the rows y) and along y (the columns
+// x) through this block and NH neighbour blocks on each side, the neighbours straight from global memory.  zq is
+// uniform within the wave, so the plane's offset goes into the (scalar) base pointers.
+template <typename real_t, int NH, bool IMG>
+__device__ __forceinline__ void plane_xy(LocalArgs<real_t> const& a, uint64_t blk, int p, int zq, uint32_t boff_p,
+                                         uint32_t boff_o, real_t const (&own)[16], real_t (&acc)[16]) {
+    constexpr int NW = 4 * (2 * NH + 1);
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        real_t const* nb[2][NH];
+        real_t fr[2][NH], fi[2][NH];
+        neighbours<real_t, NH, IMG>(a, d, blk, p, nb, fr, fi);
+        int const zqu = __builtin_amdgcn_readfirstlane(zq);
+        real_t const* nbz[2][NH];
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) nbz[sd][kk] = nb[sd][kk] + zqu * 16 * 64;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            real_t w[NW];   // the line through this block and NH neighbour blocks on each side
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);   // point of the plane on the line
+                int const e = k * 64;   // plane zq is in the base pointers
+#pragma unroll
+                for (int kk = 0; kk < NH; ++kk) {
+                    w[4 * (NH - 1 - kk) + q] = nb_value<real_t, IMG>(nbz[0][kk], e, boff_p, boff_o, fr[0][kk], fi[0][kk]);
+                    w[4 * (NH + 1 + kk) + q] = nb_value<real_t, IMG>(nbz[1][kk], e, boff_p, boff_o, fr[1][kk], fi[1][kk]);
+                }
+                w[4 * NH + q] = own[k];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);
+                stencil<real_t, NH, NW>(acc[k], a, d, w, 4 * NH + q);
+            }
+        }
+    }
+}
+
+// The end of a register-plane kernel for one block slice: the SHO add-back of the atom images (numax <= 5)
+// touching blk, the store of the thread's 16 points and, if a.dot is set, the bilinear product (x, y) of every
+// column of the slice.  cs is LDS of at least 2 * 64 * JW elements that no thread reads any more (the
+// coefficients of one image at a time, then the partial sums of the product); xo = x of the other plane at the
+// thread's points.  Contains work-group barriers: every thread of the work-group calls it.
+template <typename real_t, int JW>
+__device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_t blk, int slice, int p, int zq,
+                                             int jl, real_t (&acc)[16], real_t const (&own)[16],
+                                             real_t const (&xo)[16], real_t* cs, real_t (*H)[3][4], float (*xi2)[4]) {
+    constexpr int T = 8 * JW;
+    using vec_t = typename std::conditional<sizeof(real_t) == 8, double2, float4>::type;
+    constexpr int VE = 16 / sizeof(real_t), VPR = JW / VE;
+    int const tid = threadIdx.x, j = slice * JW + jl;
+    uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
+    if (b1 > b0) {
+        uint32_t const col = a.colindx[blk];
+        float const R2 = float(a.hgrid[3] * a.hgrid[3]);
+        for (uint32_t b = b0; b < b1; ++b) {
+            uint32_t const img = a.add_index[b];
+            int const L = a.image_lmax[img];
+            if (L > 5) continue;  // done by k_sho_add_wide
+            int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
+            __syncthreads();
+            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
+            {   // [coefficient][plane][JW]
+                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
+                for (int v = tid; v < 2 * nc * VPR; v += T) {
+                    int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
+                    *reinterpret_cast<vec_t*>(cs + row * JW + cv * VE) =
+                        *reinterpret_cast<vec_t const*>(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * VE);
+                }
+            }
+            __syncthreads();
+            sho_add_regs<real_t>(acc, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
+        }
+    }
+
+    real_t* dst = a.y + blk * NBLK + size_t(p * 4096 + j) + size_t(zq * 16) * 64;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) dst[size_t(k) * 64] = acc[k];
+    if (a.dot) {
+        // (x, y) = sum_i x_i y_i per column: plane 0 holds x_r, y_r (+ x_i from LDS), plane 1 x_i, y_i (+ x_r):
+        // Re = sum x_r y_r - sum x_i y_i, Im = sum x_i y_r + sum x_r y_i
+        real_t re_t = 0, im_t = 0;   // 16 products in the vector precision, then summed in FP64
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            re_t += own[k] * acc[k];
+            im_t += xo[k] * acc[k];
+        }
+        double re = re_t, im = im_t;
+        if (p == 1) re = -re;
+        __syncthreads();   // cs is free
+        double* red = reinterpret_cast<double*>(cs);   // [zq][p][re, im][JW]
+        red[((zq * 2 + p) * 2 + 0) * JW + jl] = re;
+        red[((zq * 2 + p) * 2 + 1) * JW + jl] = im;
+        __syncthreads();
+        if (tid < JW) {
+            double sr = 0, si = 0;
+            for (int g = 0; g < 8; ++g) { sr += red[(g * 2 + 0) * JW + tid]; si += red[(g * 2 + 1) * JW + tid]; }
+            a.dot[(blk * 2 + 0) * 64 + slice * JW + tid] = sr;
+            a.dot[(blk * 2 + 1) * 64 + slice * JW + tid] = si;
+        }
+    }
+}
+
 // Register-plane variant of the local kernel: one work-group per (block, JW-column slice), 8 JW threads; thread
 // (jl, p, zq) owns the 16 points of z plane zq of column j = slice JW + jl, plane p, and accumulates potential,
 // kinetic x, y, z and the SHO add-back in registers.  LDS holds the block slice only (read by the z-stencil and
@@ -674,7 +815,6 @@ template <typename real_t, int NH, bool IMG, int JW>
 __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     static_assert(2 * JW >= 64, "zq must be uniform within a wave");
     constexpr int T = 8 * JW;
-    constexpr int NW = 4 * (2 * NH + 1);
     constexpr int NSLICE = 64 / JW;
     __shared__ real_t xs[2 * 64 * JW];
     __shared__ real_t H[8][3][4];
@@ -713,135 +853,29 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) own[k] = xsp[(zq * 16 + k) * JW];
 
-    // ---- potential: (V - E) x, the imaginary part of E coupling the two planes ----
+    plane_potential<real_t, JW>(a, blk, p, zq, own, xso, acc);
+    plane_xy<real_t, NH, IMG>(a, blk, p, zq, boff_p, boff_o, own, acc);
     {
-        int32_t const iv = a.veff[blk];
-        real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
-        double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + zq * 16;
-        uint32_t const colE = a.colindx[blk];
-        real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
-        real_t const Vim = E_im * real_t(1 - 2 * p);
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            acc[k] = (real_t(Vb[k]) * vm + (a.cdiag - E_re)) * own[k] + Vim * xso[(zq * 16 + k) * JW];
-    }
-
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
         real_t const* nb[2][NH];
         real_t fr[2][NH], fi[2][NH];
-#pragma unroll
-        for (int sd = 0; sd < 2; ++sd)
-#pragma unroll
-            for (int k = 0; k < NH; ++k) {
-                int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
-                uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
-                nb[sd][k] = (code == -1) ? a.zero : a.x + idx * NBLK;
-                bool const img = IMG && (code <= -2);
-                fr[sd][k] = img ? real_t(a.phase[d][sd][0]) : real_t(1);
-                fi[sd][k] = img ? real_t(a.phase[d][sd][1]) * real_t(1 - 2 * p) : real_t(0);
-            }
-        if (d < 2) {
-            // lines of this plane: along x (d = 0) the rows y, along y (d = 1) the columns x; zq is uniform
-            // within the wave, so the plane's offset goes into the (scalar) base pointers
-            int const zqu = __builtin_amdgcn_readfirstlane(zq);
-            real_t const* nbz[2][NH];
-#pragma unroll
-            for (int sd = 0; sd < 2; ++sd)
-#pragma unroll
-                for (int kk = 0; kk < NH; ++kk) nbz[sd][kk] = nb[sd][kk] + zqu * 16 * 64;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                real_t w[NW];   // the line through this block and NH neighbour blocks on each side
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);   // point of the plane on the line
-                    int const e = k * 64;   // plane zq is in the base pointers
-#pragma unroll
-                    for (int kk = 0; kk < NH; ++kk) {
-                        w[4 * (NH - 1 - kk) + q] = nb_value<real_t, IMG>(nbz[0][kk], e, boff_p, boff_o, fr[0][kk], fi[0][kk]);
-                        w[4 * (NH + 1 + kk) + q] = nb_value<real_t, IMG>(nbz[1][kk], e, boff_p, boff_o, fr[1][kk], fi[1][kk]);
-                    }
-                    w[4 * NH + q] = own[k];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int const k = (d == 0) ? (r * 4 + q) : (q * 4 + r);
-                    stencil<real_t, NH, NW>(acc[k], a, d, w, 4 * NH + q);
-                }
-            }
-        } else {
-            switch (zq) {
-                case 0: z_pass<real_t, NH, IMG, JW, 0>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
-                case 1: z_pass<real_t, NH, IMG, JW, 1>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
-                case 2: z_pass<real_t, NH, IMG, JW, 2>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
-                default: z_pass<real_t, NH, IMG, JW, 3>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
-            }
+        neighbours<real_t, NH, IMG>(a, 2, blk, p, nb, fr, fi);
+        switch (zq) {
+            case 0: z_pass<real_t, NH, IMG, JW, 0>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
+            case 1: z_pass<real_t, NH, IMG, JW, 1>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
+            case 2: z_pass<real_t, NH, IMG, JW, 2>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
+            default: z_pass<real_t, NH, IMG, JW, 3>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
         }
     }
 
-    // the other plane of x at this thread's points, for the bilinear product (x, y) of the epilogue
+    // the other plane of x at this thread's points, for the bilinear product (x, y) of the epilogue, before the
+    // add-back takes over the LDS of the slice
     real_t xo[16];
     if (a.dot) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) xo[k] = xso[(zq * 16 + k) * JW];
     }
-
-    // ---- SHO add-back of the atom images (numax <= 5) touching this block; the coefficients of each image are
-    //      staged in the LDS buffer of the block slice, which is free after the kinetic passes ----
-    uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
-    if (b1 > b0) {
-        uint32_t const col = a.colindx[blk];
-        float const R2 = float(a.hgrid[3] * a.hgrid[3]);
-        real_t* cs = xs;    // [nc][2][JW]
-        for (uint32_t b = b0; b < b1; ++b) {
-            uint32_t const img = a.add_index[b];
-            int const L = a.image_lmax[img];
-            if (L > 5) continue;  // done by k_sho_add_wide
-            int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
-            __syncthreads();
-            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
-            {
-                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
-                for (int v = tid; v < 2 * nc * VPR; v += T) {
-                    int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
-                    *reinterpret_cast<vec_t*>(cs + row * JW + cv * VE) =
-                        *reinterpret_cast<vec_t const*>(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * VE);
-                }
-            }
-            __syncthreads();
-            sho_add_regs<real_t>(acc, H, xi2, R2, cs + p * JW + jl, 2 * JW, zq, L);
-        }
-    }
-
-    real_t* dst = a.y + blk * NBLK + size_t(p * 4096 + j) + size_t(zq * 16) * 64;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) dst[size_t(k) * 64] = acc[k];
-    if (a.dot) {
-        // (x, y) = sum_i x_i y_i per column: plane 0 holds x_r, y_r (+ x_i from LDS), plane 1 x_i, y_i (+ x_r):
-        // Re = sum x_r y_r - sum x_i y_i, Im = sum x_i y_r + sum x_r y_i
-        real_t re_t = 0, im_t = 0;   // 16 products in the vector precision, then summed in FP64
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            re_t += own[k] * acc[k];
-            im_t += xo[k] * acc[k];
-        }
-        double re = re_t, im = im_t;
-        if (p == 1) re = -re;
-        __syncthreads();   // the LDS of the slice is free
-        double* red = reinterpret_cast<double*>(xs);   // [zq][p][re, im][JW]
-        red[((zq * 2 + p) * 2 + 0) * JW + jl] = re;
-        red[((zq * 2 + p) * 2 + 1) * JW + jl] = im;
-        __syncthreads();
-        if (tid < JW) {
-            double sr = 0, si = 0;
-            for (int g = 0; g < 8; ++g) { sr += red[(g * 2 + 0) * JW + tid]; si += red[(g * 2 + 1) * JW + tid]; }
-            a.dot[(blk * 2 + 0) * 64 + slice * JW + tid] = sr;
-            a.dot[(blk * 2 + 1) * 64 + slice * JW + tid] = si;
-        }
-    }
+    plane_finish<real_t, JW>(a, blk, slice, p, zq, jl, acc, own, xo, xs, H, xi2);
 }
-
 
 // SHO add-back of atom images with numax 6 or 7, y += P d, for the (block, image) pairs listed in
 // pairs[]: one work-group of 256 threads = (column j, plane p, z plane zq) per pair.  Runs after k_local.
@@ -989,6 +1023,7 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
             }
         }
         if ((s = dupload(d, d.nbr, nb, err))) return s;
+
     }
 
     // dyadic structures
