@@ -683,15 +683,16 @@ __device__ __forceinline__ void neighbours(LocalArgs<real_t> const& a, int d, ui
         }
 }
 
-// (V - E) x at the thread's 16 points (z plane zq of one column, plane p): own = x of plane p there, xso = x of
-// the other plane in LDS (row stride JW), the imaginary part of E coupling the two planes; the centre weights of
-// the three finite-difference directions are applied here as well
+// (V - E) x at the thread's 16 points (z plane zq of one column, plane p; zq uniform within the wave): own = x of
+// plane p there, xso = x of the other plane in LDS (row stride JW), the imaginary part of E coupling the two
+// planes; the centre weights of the three finite-difference directions are applied here as well
 template <typename real_t, int JW>
 __device__ __forceinline__ void plane_potential(LocalArgs<real_t> const& a, uint64_t blk, int p, int zq,
                                                 real_t const (&own)[16], real_t const* xso, real_t (&acc)[16]) {
     int32_t const iv = a.veff[blk];
     real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
-    double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + zq * 16;
+    // zq is uniform within the wave: the 16 potential values are scalar loads
+    double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + __builtin_amdgcn_readfirstlane(zq) * 16;
     uint32_t const colE = a.colindx[blk];
     real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
     real_t const Vim = E_im * real_t(1 - 2 * p);
@@ -877,30 +878,30 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     plane_finish<real_t, JW>(a, blk, slice, p, zq, jl, acc, own, xo, xs, H, xi2);
 }
 
-// SHO add-back of atom images with numax 6 or 7, y += P d, for the (block, image) pairs listed in
-// pairs[]: one work-group of 256 threads = (column j, plane p, z plane zq) per pair.  Runs after k_local.
+// SHO add-back of the atom images with numax 6 or 7, y += P d, on the blocks listed in blocks[]: one work-group of
+// 512 threads = (column j, plane p, z plane zq) per block, taking the block's wide images one after the other, so
+// that no two work-groups update the same block.  Runs after the local kernel (which adds numax <= 5).
 template <typename real_t>
-__global__ __launch_bounds__(512) void k_sho_add_wide(LocalArgs<real_t> a, uint32_t const* __restrict__ pairs,
+__global__ __launch_bounds__(512) void k_sho_add_wide(LocalArgs<real_t> a, uint32_t const* __restrict__ blocks,
                                                       uint8_t const* __restrict__ col_active) {
     __shared__ real_t H[8][3][4];
     __shared__ float xi2[3][4];
-    __shared__ double apos[4];
-    __shared__ float cpos[4];
-    __shared__ double hgrid[4];
-    uint32_t const blk = pairs[2 * blockIdx.x], img = pairs[2 * blockIdx.x + 1];
-    if (col_active && !col_active[a.colindx[blk]]) return;
+    uint32_t const blk = blocks[blockIdx.x];
+    uint32_t const col = a.colindx[blk];
+    if (col_active && !col_active[col]) return;
     int const tid = threadIdx.x, j = tid & 63, p = (tid >> 6) & 1, zq = tid >> 7;
-    int const L = a.image_lmax[img];
-    if (tid < 4) { apos[tid] = a.image_pos[4 * size_t(img) + tid]; hgrid[tid] = a.hgrid[tid]; }
-    if (tid < 3) cpos[tid] = a.row_pos[4 * size_t(a.rowindx[blk]) + tid];
-    __syncthreads();
-    if (tid < 12) hermite_1d<real_t>(H, xi2, tid, L, apos, cpos, hgrid);
-    __syncthreads();
     float const R2 = float(a.hgrid[3] * a.hgrid[3]);
-    real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + a.colindx[blk]) * 128 + p * 64 + j;
-    real_t* yb = a.y + size_t(blk) * NBLK + p * 4096 + j;
-    // ys stride 64 for the direct update of y
-    sho_add_atom<real_t, 7>(yb, 64, H, xi2, R2, dsrc, size_t(a.nCols) * 128, zq, L);
+    real_t* yb = a.y + size_t(blk) * NBLK + p * 4096 + j;   // ys stride 64 for the direct update of y
+    for (uint32_t b = a.add_start[blk]; b < a.add_start[blk + 1]; ++b) {
+        uint32_t const img = a.add_index[b];
+        int const L = a.image_lmax[img];
+        if (L <= 5) continue;   // added by the local kernel
+        __syncthreads();        // H of the previous image is no longer read
+        stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, 512);
+        __syncthreads();
+        real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + p * 64 + j;
+        sho_add_atom<real_t, 7>(yb, 64, H, xi2, R2, dsrc, size_t(a.nCols) * 128, zq, L);
+    }
 }
 
 template <typename T>
@@ -1029,13 +1030,13 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
     // dyadic structures
     if ((s = dupload(d, d.add_start, p.sho_add.start, err))) return s;
     if ((s = dupload(d, d.add_index, p.sho_add.index, err))) return s;
-    {   // (block, image) pairs with numax > 5, added by a separate kernel after k_local
+    {   // blocks touched by atom images with numax > 5, whose add-back runs in a separate kernel after k_local
         std::vector<uint32_t> wide;
         for (uint64_t blk = 0; blk < p.nnzb; ++blk)
             for (uint32_t k = p.sho_add.start[blk]; k < p.sho_add.start[blk + 1]; ++k)
-                if (p.image_lmax[p.sho_add.index[k]] > 5) { wide.push_back(uint32_t(blk)); wide.push_back(p.sho_add.index[k]); }
-        d.n_wide_pairs = uint32_t(wide.size() / 2);
-        if ((s = dupload(d, d.wide_pairs, wide, err))) return s;
+                if (p.image_lmax[p.sho_add.index[k]] > 5) { wide.push_back(uint32_t(blk)); break; }
+        d.n_wide_blocks = uint32_t(wide.size());
+        if ((s = dupload(d, d.wide_blocks, wide, err))) return s;
     }
     {   // (image, target row) slots of the Hermite table, for the add lists and the projection chunks below
         std::map<uint64_t, uint32_t> slot_of;
@@ -1356,8 +1357,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
         else launch_local<real_t, 1, false>(KIND, JW, LB, g, stream, a);
     }
     HIP_TRY(hipGetLastError());
-    if (d.n_wide_pairs > 0) {
-        hipLaunchKernelGGL((k_sho_add_wide<real_t>), dim3(d.n_wide_pairs), dim3(512), 0, stream, a, d.wide_pairs,
+    if (d.n_wide_blocks > 0) {
+        hipLaunchKernelGGL((k_sho_add_wide<real_t>), dim3(d.n_wide_blocks), dim3(512), 0, stream, a, d.wide_blocks,
                            d.col_active);
         HIP_TRY(hipGetLastError());
     }

@@ -69,8 +69,8 @@ struct DevicePlan {
     float* row_pos = nullptr;          // [nRows][4]
     uint32_t* add_start = nullptr;     // [nnzb+1]
     uint32_t* add_index = nullptr;
-    uint32_t* wide_pairs = nullptr;    // (block, image) pairs with numax > 5
-    uint32_t n_wide_pairs = 0;
+    uint32_t* wide_blocks = nullptr;   // blocks touched by an atom image with numax > 5
+    uint32_t n_wide_blocks = 0;
     double* image_pos = nullptr;       // [nai][4]
     int8_t* image_lmax = nullptr;      // [nai]
     uint32_t* image_starts = nullptr;  // [nai+1]

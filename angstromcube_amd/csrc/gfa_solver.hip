@@ -1199,7 +1199,7 @@ int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p, T* q, hipStr
     dim3 const gb(st.nseg), tb(256), gs(st.nCols), ts(64 * FOLD_P);
     // (p, A p) from the action's epilogue when nothing is added to A p after the local kernel (no numax 6, 7
     // atoms), else from a pass over p and q
-    bool const fused = capi_dev(plan).n_wide_pairs == 0;
+    bool const fused = capi_dev(plan).n_wide_blocks == 0;
     if (int e = capi_action(plan, fp, q, p, s, err, ACTION_ALL, fused ? st.bpart : nullptr)) return e;
     if (fused) hipLaunchKernelGGL(k_fold_blocks, gb, dim3(64), 0, s, st.bpart, g);
     else hipLaunchKernelGGL((k_cocg_pq<T>), gb, tb, 0, s, p, q, g);
