@@ -11,13 +11,14 @@
 //   2. k_sho_atoms    collect images -> atom with Bloch phases, multiply the atom matrix
 //                     (h - E s) dVol, broadcast back with conjugate phases
 //                     (include/green_dyadic.hxx:600-852, SHOsum + SHOmul + SHOsum);
-//   3. k_local        per block: (V - E) x (include/green_potential.hxx:16-139) + the three
-//                     finite-difference Laplacians (include/green_kinetic.hxx:24-329) + the SHO
-//                     add-back of all atoms touching the block (include/green_dyadic.hxx:325-518),
-//                     written once.  The block is staged in LDS; neighbour blocks come through L2.
+//   3. the local kernel, per (block, column slice): (V - E) x (include/green_potential.hxx:16-139) + the
+//                     three finite-difference Laplacians (include/green_kinetic.hxx:24-329) + the SHO add-back
+//                     of the atoms touching the block with numax <= 5 (include/green_dyadic.hxx:325-518),
+//                     written once.  Two kinds, picked per plan and precision by the autotuner: k_local (LDS
+//                     accumulator) and k_plane (register accumulator per z plane).  The slice is staged in
+//                     LDS; neighbour blocks come through L2.  Images with numax 6, 7 are added afterwards by
+//                     k_sho_add_wide.
 #include "gfa_device.hpp"
-
-
 
 #include <algorithm>
 #include <map>
@@ -262,10 +263,10 @@ __global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
 }
 
 // ---------------------------------------------------------------------------------------------
-// 3. local part of the action: one work-group per (block, half of the 64 columns), 128 threads =
-//    (column j of the half, plane p, z half hz); each thread produces 32 points of one (j, p).
-//    The block's column half is staged in LDS; neighbour blocks are read with branch-free addressing
-//    (an absent neighbour reads a zero block) so that all loads of a line are in flight together.
+// 3. local part of the action: one work-group per (block, slice of JW of the 64 columns), 8 JW threads, each
+//    owning points of one (column, plane).  The slice is staged in LDS; neighbour blocks are read with
+//    branch-free addressing (an absent neighbour reads a zero block) so that all loads of a line are in flight
+//    together.
 // ---------------------------------------------------------------------------------------------
 template <typename real_t>
 struct LocalArgs {
