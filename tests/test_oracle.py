@@ -216,3 +216,16 @@ def test_oracle_tfqmr_matches_dense_solve():
         xd = np.linalg.solve(A, b)
         xt = (X[blocks, 0, :, j] + 1j * X[blocks, 1, :, j]).reshape(-1)
         assert np.abs(xt - xd).max() < 1e-8 * np.abs(xd).max()
+
+
+@needs_ref
+@pytest.mark.parametrize("name,mk,kw,E,k", [
+    ("periodic4_numax6_k", lambda: S.make_system(0xA43C0006, (4, 4, 4), ngauss=4, natoms=3, numax=6),
+     dict(r_trunc=6.0), complex(-0.4, 0.1), (0.25, 0.0, -0.125)),
+    ("isolated5_numax7", lambda: S.make_system(0xA43C0007, (5, 5, 5), ngauss=4, natoms=3, numax=7, bc=(0, 0, 0)),
+     dict(r_trunc=2.5, nsources=(2, 1, 1)), complex(-0.3, 0.2), (0.0, 0.0, 0.0)),
+])
+def test_oracle_equals_reference_action_numax_6_7(name, mk, kw, E, k):
+    """The dyadic term with 84 and 120 SHO coefficients per atom (the reference's largest, Lmax 7), against the
+    reference's own CPU action: the geometries of the GPU tests of these atoms."""
+    test_oracle_equals_reference_action(name, mk, kw, E, k)
