@@ -31,7 +31,7 @@ namespace {
 
 constexpr int NBLK = 2 * 64 * 64;  // elements of one block
 #ifndef LOCAL_TILE
-#define LOCAL_TILE 8   // xy tile edge (blocks) of the local kernel's processing order
+#define LOCAL_TILE 6   // xy tile edge (blocks) of the local kernel's processing order (5 to 7 measured best)
 #endif
 
 #define HIP_TRY(call)                                                              \
