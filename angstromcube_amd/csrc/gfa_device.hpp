@@ -48,8 +48,9 @@ struct DevicePlan {
     int nh_max = 1;                    // neighbour blocks per side in the widest stencil
     bool has_images = false;           // some finite-difference halo is a periodic image
     int fd_range[3] = {4, 4, 4};
-    // k_local tile shape per precision (index 0: FP64, 1: FP32), set by the autotuner
-    int tune_kind[2] = {0, 0}, tune_jw[2] = {32, 32}, tune_lb[2] = {2, 2};
+    // local-kernel tile shape per precision (index 0: FP64, 1: FP32), set by the autotuner; the defaults are the
+    // shapes measured fastest on a 159,256-block plan (FP64 k_local 16 columns x 2 lines, FP32 k_plane 32 columns)
+    int tune_kind[2] = {0, 1}, tune_jw[2] = {16, 32}, tune_lb[2] = {2, 0};
     bool tuned[2] = {false, false};
 
     // index data
