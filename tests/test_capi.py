@@ -40,6 +40,18 @@ def test_plan_without_gpu_and_queries():
     del G_wrap
 
 
+def test_default_and_pinned_tile_shapes_without_gpu():
+    """A new plan reports the default local-kernel shapes (FP64 k_local 16 columns x 2 lines, FP32 k_plane 32
+    columns) as not tuned; pinning records the shape as tuned, and unknown shapes are refused."""
+    p = G.GreenPlan(S.small_periodic(), r_trunc=6.0)
+    assert p.tile_shape(64) == (0, 16, 2, 0) and p.tile_shape(32) == (1, 32, 0, 0)
+    p.set_tile_shape(64, 1, 64)
+    assert p.tile_shape(64) == (1, 64, 0, 1)
+    for bad in ((0, 24, 2), (1, 16, 0), (2, 32, 0)):
+        with pytest.raises(G.GfaError, match="tile"):
+            p.set_tile_shape(32, *bad)
+
+
 @pytest.mark.parametrize("bad,msg", [
     (dict(ng=(30, 32, 32)), "multiple of 4"),
     (dict(r_trunc=-1.0), "non-negative"),
