@@ -1377,12 +1377,20 @@ int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t con
     int const v = sizeof(real_t) == 8 ? 0 : 1;
     static int const SHAPES[8][3] = {{0, 16, 1}, {0, 16, 2}, {0, 16, 4}, {0, 32, 1}, {0, 32, 2}, {0, 32, 4},
                                      {1, 32, 0}, {1, 64, 0}};
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
     double best = 1e300;
     int bkind = d.tune_kind[v], bjw = d.tune_jw[v], blb = d.tune_lb[v];
-    for (auto const& sh : SHAPES) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // the shape in use before the call (or the fastest so far) is restored on every return, also on errors
+    auto finish = [&](int status) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        d.tune_kind[v] = bkind; d.tune_jw[v] = bjw; d.tune_lb[v] = blb;
+        return status;
+    };
+    auto timed = [&]() -> int {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        for (auto const& sh : SHAPES) {
             int const kind = sh[0], jw = sh[1], lb = sh[2];
             d.tune_kind[v] = kind; d.tune_jw[v] = jw; d.tune_lb[v] = lb;
             if (int r = device_action<real_t>(d, s, y, x, stream, err)) return r;  // warm-up
@@ -1395,13 +1403,13 @@ int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t con
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
             double const t = double(ms) / std::max(1, reps);
             if (t < best) { best = t; bkind = kind; bjw = jw; blb = lb; }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    d.tune_kind[v] = bkind; d.tune_jw[v] = bjw; d.tune_lb[v] = blb;
+        }
+        return 0;
+    };
+    if (int r = timed()) return finish(r);
     d.tuned[v] = true;
     if (best_ms) *best_ms = best;
-    return 0;
+    return finish(0);
 }
 template int device_autotune<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, int, double*, std::string&);
 template int device_autotune<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, int, double*, std::string&);

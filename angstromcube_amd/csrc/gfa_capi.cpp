@@ -162,6 +162,7 @@ int gfa_plan_query(const gfa_plan* plan, gfa_sizes* s) {
         s->sho_sum_nnz = h.sho_sum.index.size();
         s->nchunks = plan->dev.nchunks;
         s->device_bytes = plan->dev.bytes;
+        s->solver_bytes = gfa::solver_held_bytes(const_cast<gfa_plan*>(plan));
         return GFA_OK;
     });
 }
@@ -504,6 +505,51 @@ int gfa_action_multiply(gfa_plan* plan, int fp_bits, void* y, const void* x, con
     }
     if (precondition) { g_err = "the action has no preconditioner"; return GFA_ERR_ARGUMENT; }
     return gfa_action(plan, fp_bits, y, x, stream, nflops);
+}
+
+int gfa_solver_release(gfa_plan* plan) {
+    return guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        gfa::solver_release(plan);
+        return GFA_OK;
+    });
+}
+
+int gfa_solve_memory(gfa_plan* plan, const gfa_solve_opts* opts, uint64_t* solver_bytes, uint64_t* plan_bytes) {
+    return guarded([&]() -> int {
+        if (!plan || !opts) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (opts->fp_bits != 64 && opts->fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (solver_bytes) *solver_bytes = gfa::solve_bytes(plan, opts->fp_bits, opts->method);
+        if (plan_bytes) {
+            if (plan->dev.uploaded) {
+                *plan_bytes = plan->dev.bytes;
+            } else {   // index data (about 100 bytes per block and 12 per add/projection entry) and the potential
+                gfa::Plan const& h = plan->host;
+                uint64_t b = h.nnzb * 120 + h.ncell() * 64 * 8 + h.atom_hs.size() * 8;
+                b += h.sho_add.index.size() * 12;
+                for (auto const& c : h.sho_prj) b += c.index.size() * 12;
+                b += uint64_t(h.natomcoeffs()) * h.nCols * 128 * 8 * 2;   // coefficient workspace (both precisions)
+                *plan_bytes = b;
+            }
+        }
+        return GFA_OK;
+    });
+}
+
+int gfa_plan_prepare(gfa_plan* plan, int fp_bits) {
+    return guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energy before preparing the plan");
+        if (int s = ensure_device(plan)) return s;
+        std::string err;
+        // parts = 0 allocates the action's workspace of this precision and launches nothing
+        if (int s = gfa::capi_action(plan, fp_bits, nullptr, nullptr, nullptr, err, 0))
+            return fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
+        hipError_t const e = hipDeviceSynchronize();
+        if (e != hipSuccess) return fail(GFA_ERR_DEVICE, std::string("prepare: ") + hipGetErrorString(e));
+        return GFA_OK;
+    });
 }
 
 int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve_stats* stats) {

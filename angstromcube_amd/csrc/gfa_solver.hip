@@ -165,11 +165,27 @@ __device__ __forceinline__ void coefs(Cx<T> (&c)[N], double const* __restrict__ 
     (void)q0; (void)r;
 #define OFF(p, i) ((uint64_t(blk) * 2 + (p)) * 4096 + uint64_t(i) * 64 + j0)
 
-// x untouched; d = 0, w = u = b (unit columns on the diagonal blocks if rhs is null);
-// partials |b|^2 and (r0*, b)
+// the right-hand side of an inner solve at the thread's V columns of row i: the unit columns on the diagonal blocks
+// minus y = A x (the FP64 action on the current FP64 solution, so b - A x is the refinement residual), or the unit
+// columns alone if y is null, rounded to T
+template <typename T, int V>
+__device__ __forceinline__ void rhs_values(T (&br)[V], T (&bi)[V], double const* __restrict__ y, uint64_t o_re,
+                                           uint64_t o_im, bool dg, int i, int j0) {
+    if (y) {
+        double yr[V], yi[V];
+        ldd<V>(yr, y + o_re); ldd<V>(yi, y + o_im);
+#pragma unroll
+        for (int t = 0; t < V; ++t) { br[t] = T(((dg && i == j0 + t) ? 1.0 : 0.0) - yr[t]); bi[t] = T(-yi[t]); }
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; ++t) { br[t] = (dg && i == j0 + t) ? T(1) : T(0); bi[t] = 0; }
+    }
+}
+
+// x untouched; d = 0, w = u = b (see rhs_values); partials |b|^2 and (r0*, b)
 template <typename T>
 __global__ __launch_bounds__(256) void k_init(T* __restrict__ d, T* __restrict__ w, T* __restrict__ u,
-                                              T const* __restrict__ rhs, uint8_t const* __restrict__ diag,
+                                              double const* __restrict__ y, uint8_t const* __restrict__ diag,
                                               SegArgs g, uint64_t seed) {
     SEG_PROLOGUE
     double acc[3][V] = {};
@@ -180,11 +196,7 @@ __global__ __launch_bounds__(256) void k_init(T* __restrict__ d, T* __restrict__
         for (int ps = 0; ps < M::NPASS; ++ps) {
             int const i = ps * M::RPP + r;
             T br[V], bi[V], z[V];
-            if (rhs) { ld(br, rhs + OFF(0, i)); ld(bi, rhs + OFF(1, i)); }
-            else {
-#pragma unroll
-                for (int t = 0; t < V; ++t) { br[t] = (dg && i == j0 + t) ? T(1) : T(0); bi[t] = 0; }
-            }
+            rhs_values<T, V>(br, bi, y, OFF(0, i), OFF(1, i), dg, i, j0);
 #pragma unroll
             for (int t = 0; t < V; ++t) z[t] = 0;
             st(d + OFF(0, i), z); st(d + OFF(1, i), z);
@@ -392,10 +404,9 @@ __global__ __launch_bounds__(256) void k_zero_seg(double* __restrict__ x, SegArg
     }
 }
 
-// r = b - y (FP64, b = unit columns on the diagonal blocks); partial |r|^2; optionally r_out = r
+// partial |b - y|^2 (FP64, b = unit columns on the diagonal blocks, y = A x)
 template <typename T>
-__global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, uint8_t const* __restrict__ diag,
-                                                  T* __restrict__ r_out, SegArgs g) {
+__global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, uint8_t const* __restrict__ diag, SegArgs g) {
     SEG_PROLOGUE
     double acc[1][V] = {};
     for (uint32_t b = b0; b < b1; ++b) {
@@ -406,14 +417,11 @@ __global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, 
             int const i = ps * M::RPP + r;
             double yr[V], yi[V];
             ldd<V>(yr, y + OFF(0, i)); ldd<V>(yi, y + OFF(1, i));
-            T orr[V], oi[V];
 #pragma unroll
             for (int t = 0; t < V; ++t) {
                 double const rr = ((dg && i == j0 + t) ? 1.0 : 0.0) - yr[t], ri = -yi[t];
                 acc[0][t] += rr * rr + ri * ri;
-                orr[t] = T(rr); oi[t] = T(ri);
             }
-            if (r_out) { st(r_out + OFF(0, i), orr); st(r_out + OFF(1, i), oi); }
         }
     }
     store_partials<T, 1>(acc, g.segpart, s);
@@ -423,9 +431,9 @@ __global__ __launch_bounds__(256) void k_residual(double const* __restrict__ y, 
 // COCG (conjugate orthogonal CG for complex-symmetric A, van der Vorst & Melissen, IEEE Trans. Magn. 26, 706
 // (1990)): CG with the bilinear form (a, b) = sum a_i b_i.  Vectors r, p, q = A p; the solution in FP64.
 // ------------------------------------------------------------------------------------------------
-// r = p = b; partials |b|^2, Re (b, b), Im (b, b)
+// r = p = b (see rhs_values); partials |b|^2, Re (b, b), Im (b, b)
 template <typename T>
-__global__ __launch_bounds__(256) void k_cocg_init(T* __restrict__ rv, T* __restrict__ p, T const* __restrict__ rhs,
+__global__ __launch_bounds__(256) void k_cocg_init(T* __restrict__ rv, T* __restrict__ p, double const* __restrict__ y,
                                                    uint8_t const* __restrict__ diag, SegArgs g) {
     SEG_PROLOGUE
     double acc[3][V] = {};
@@ -436,11 +444,7 @@ __global__ __launch_bounds__(256) void k_cocg_init(T* __restrict__ rv, T* __rest
         for (int ps = 0; ps < M::NPASS; ++ps) {
             int const i = ps * M::RPP + r;
             T br[V], bi[V];
-            if (rhs) { ld(br, rhs + OFF(0, i)); ld(bi, rhs + OFF(1, i)); }
-            else {
-#pragma unroll
-                for (int t = 0; t < V; ++t) { br[t] = (dg && i == j0 + t) ? T(1) : T(0); bi[t] = 0; }
-            }
+            rhs_values<T, V>(br, bi, y, OFF(0, i), OFF(1, i), dg, i, j0);
             st(rv + OFF(0, i), br); st(rv + OFF(1, i), bi);
             st(p + OFF(0, i), br); st(p + OFF(1, i), bi);
 #pragma unroll
@@ -941,15 +945,15 @@ struct SolverState {
     std::vector<double> col_halfsteps;      // [ncol] half-steps of every column in this solve
     // vectors
     double* x64 = nullptr;                  // the FP64 solution
-    double* rhs64 = nullptr;                // FP64 residual (refinement right-hand side)
-    float* rhs32 = nullptr;
-    double* work64 = nullptr;               // 7 FP64 work vectors, contiguous
-    float* work32 = nullptr;                // 7 FP32 work vectors, contiguous
+    double* y64 = nullptr;                  // A x64 with the FP64 action (the refinement right-hand side is b - y64)
+    void* arena = nullptr;                  // the inner solver's work vectors, sized for the largest method run so
+    size_t arena_bytes = 0;                 // far; FP64 and FP32 inner solves run one after another and share it
     // captured step
     hipGraphExec_t graph = nullptr;
     int graph_fp = 0;
     uint64_t graph_seed = 0;
     std::vector<void*> allocations;
+    size_t held_bytes = 0;                  // device bytes of allocations[]
 };
 
 namespace {
@@ -964,11 +968,23 @@ SolverState* state_of(gfa_plan* p, bool create) {
     return s;
 }
 
+// hipMalloc with the out-of-memory case reported as GFA_ERR_MEMORY
+int dmalloc(void** q, size_t bytes, char const* what, std::string& err) {
+    hipError_t const e = hipMalloc(q, bytes);
+    if (e == hipSuccess) return 0;
+    (void)hipGetLastError();
+    *q = nullptr;
+    err = std::string("solver ") + what + ": hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? GFA_ERR_MEMORY : GFA_ERR_DEVICE;
+}
+
 template <typename T>
 int walloc(SolverState& st, T*& ptr, size_t n, std::string& err) {
     void* q = nullptr;
-    HIP_TRY(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    size_t const bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (int r = dmalloc(&q, bytes, "state", err)) return r;
     st.allocations.push_back(q);
+    st.held_bytes += bytes;
     ptr = (T*)q;
     return 0;
 }
@@ -1026,6 +1042,7 @@ int setup(gfa_plan* plan, SolverState& st, std::string& err) {
     if ((s = walloc(st, st.bpart, size_t(h.nnzb) * 128, err))) return s;
     HIP_TRY(hipHostMalloc((void**)&st.S_host, size_t(st.ncol) * sizeof(double), hipHostMallocDefault));
     if ((s = walloc(st, st.x64, h.nnzb * NBLK, err))) return s;
+    if ((s = walloc(st, st.y64, h.nnzb * NBLK, err))) return s;
     st.ready = true;
     return 0;
 }
@@ -1062,14 +1079,24 @@ struct ActiveScope {
     ~ActiveScope() { d.active_order = nullptr; d.n_active = 0; d.col_active = nullptr; }
 };
 
-template <typename T> T* work(SolverState& st, int k);
-template <> double* work<double>(SolverState& st, int k) { return st.work64 + size_t(k) * st.nnzb * NBLK; }
-template <> float* work<float>(SolverState& st, int k) { return st.work32 + size_t(k) * st.nnzb * NBLK; }
+// work vector k of precision T in the arena
+template <typename T> T* work(SolverState& st, int k) { return reinterpret_cast<T*>(st.arena) + size_t(k) * st.nnzb * NBLK; }
 
+// work vectors per inner solve: TFQMR w, u, un, v, d, A un, A u; COCG r, p, A p and, in FP32, the FP32 increment of x
+template <typename T> int work_vectors(int method) { return method == 1 ? (sizeof(T) == 4 ? 4 : 3) : 7; }
+
+// an arena of at least n work vectors of T; a larger one replaces the old (whose contents are not kept: every
+// inner solve starts from its right-hand side), and a captured graph that points into it is dropped
 template <typename T>
-int ensure_work(SolverState& st, std::string& err) {
-    if (sizeof(T) == 8 && !st.work64) return walloc(st, st.work64, 7 * st.nnzb * NBLK, err);
-    if (sizeof(T) == 4 && !st.work32) return walloc(st, st.work32, 7 * st.nnzb * NBLK, err);
+int ensure_work(SolverState& st, int n, std::string& err) {
+    size_t const need = size_t(n) * st.nnzb * NBLK * sizeof(T);
+    if (st.arena_bytes >= need) return 0;
+    drop_graph(st);
+    if (st.arena) { HIP_TRY(hipStreamSynchronize(st.stream)); HIP_TRY(hipFree(st.arena)); }
+    st.arena = nullptr;
+    st.arena_bytes = 0;
+    if (int r = dmalloc(&st.arena, need, "work vectors", err)) return r;
+    st.arena_bytes = need;
     return 0;
 }
 
@@ -1159,12 +1186,13 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
     return 0;
 }
 
-// TFQMR on T vectors for the right-hand side rhs (nullptr: unit columns), accumulating into st.x64.
+// TFQMR on T vectors for the right-hand side b - rhs_y (rhs_y nullptr: b, the unit columns), accumulating into
+// st.x64.
 // Column groups whose columns all stopped are retired from the active set as the solve goes.
 template <typename T>
-int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
+int tfqmr(gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, double const* tol_abs, int maxit,
           int check_every, uint64_t seed, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
-    if (int s = ensure_work<T>(st, err)) return s;
+    if (int s = ensure_work<T>(st, work_vectors<T>(0), err)) return s;
     Vecs<T> V{work<T>(st, 0), work<T>(st, 1), work<T>(st, 2), work<T>(st, 3), work<T>(st, 4), work<T>(st, 5), work<T>(st, 6)};
     int const fp = sizeof(T) == 8 ? 64 : 32;
     hipStream_t const s = st.stream;
@@ -1174,7 +1202,7 @@ int tfqmr(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double 
     ActiveScope scope(dev, st);
     if (st.nseg == 0) return 0;
 
-    hipLaunchKernelGGL((k_init<T>), dim3(st.nseg), dim3(256), 0, s, V.d, V.w, V.u, rhs, st.diag, g, seed);
+    hipLaunchKernelGGL((k_init<T>), dim3(st.nseg), dim3(256), 0, s, V.d, V.w, V.u, rhs_y, st.diag, g, seed);
     hipLaunchKernelGGL(k_scalar_init, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol, tol_rel, tol_abs);
     // v = A u0 = Au_even of the first step
     if (int r = capi_action(plan, fp, V.v, V.u, s, err)) return r;
@@ -1211,17 +1239,17 @@ int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p, T* q, hipStr
     return 0;
 }
 
-// COCG on T vectors for the right-hand side rhs (nullptr: unit columns), accumulating into st.x64; converges on
-// the recursive residual |r| <= max(tol_rel |b|, tol_abs).  Requires a complex-symmetric operator.
+// COCG on T vectors for the right-hand side b - rhs_y (rhs_y nullptr: b, the unit columns), accumulating into
+// st.x64; converges on the recursive residual |r| <= max(tol_rel |b|, tol_abs).  Requires a complex-symmetric
+// operator.
 template <typename T>
-int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
+int cocg(gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, double const* tol_abs, int maxit,
          int check_every, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
-    if (int s = ensure_work<T>(st, err)) return s;
-    // 0, 1: the FP64 residual's storage (FP32); r, p, q; x = the FP32 increment of the solution (FP32) or the
-    // FP64 solution itself
-    T *r = work<T>(st, 2), *p = work<T>(st, 3), *q = work<T>(st, 4);
+    if (int s = ensure_work<T>(st, work_vectors<T>(1), err)) return s;
+    // r, p, q; x = the FP32 increment of the solution (FP32) or the FP64 solution itself
+    T *r = work<T>(st, 0), *p = work<T>(st, 1), *q = work<T>(st, 2);
     bool const f32 = sizeof(T) == 4;
-    T* x = f32 ? work<T>(st, 5) : reinterpret_cast<T*>(st.x64);
+    T* x = f32 ? work<T>(st, 3) : reinterpret_cast<T*>(st.x64);
     int const fp = f32 ? 32 : 64;
     hipStream_t const s = st.stream;
     DevicePlan& dev = capi_dev(plan);
@@ -1230,7 +1258,7 @@ int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double c
     std::vector<uint8_t> const started = st.active;
     {
         SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
-        hipLaunchKernelGGL((k_cocg_init<T>), dim3(st.nseg), dim3(256), 0, s, r, p, rhs, st.diag, g);
+        hipLaunchKernelGGL((k_cocg_init<T>), dim3(st.nseg), dim3(256), 0, s, r, p, rhs_y, st.diag, g);
         hipLaunchKernelGGL(k_cocg_scalar_init, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.S, st.segpart, st.col_seg,
                            st.col_active, st.ncol, tol_rel, tol_abs);
         if (f32) hipLaunchKernelGGL((k_zero_x<T>), dim3(st.nseg), dim3(256), 0, s, x, g);
@@ -1253,7 +1281,7 @@ int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double c
     // of the true residual; b is the unit columns also in refinement solves, since x holds the whole solution.
     constexpr int REPLACE = 64;
     int last = 0;
-    double* y64 = reinterpret_cast<double*>(work<T>(st, 0));   // spans work vectors 0 and 1 for FP32
+    double* y64 = st.y64;   // A x64, replaced for the active groups (their right-hand side is no longer read)
     auto hook = [&](int m) -> int {
         if (!f32 || m - last < REPLACE) return 0;
         last = m;
@@ -1279,26 +1307,25 @@ int cocg(gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double c
     return 0;
 }
 
-// the inner solver of `method` (0 TFQMR, 1 COCG)
+// the inner solver of `method` (0 TFQMR, 1 COCG) for the right-hand side b - rhs_y
 template <typename T>
-int krylov(int method, gfa_plan* plan, SolverState& st, T const* rhs, double tol_rel, double const* tol_abs, int maxit,
-           int check_every, uint64_t seed, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
-    if (method == 1) return cocg<T>(plan, st, rhs, tol_rel, tol_abs, maxit, check_every, eager, autotune, res, err);
-    return tfqmr<T>(plan, st, rhs, tol_rel, tol_abs, maxit, check_every, seed, eager, autotune, res, err);
+int krylov(int method, gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, double const* tol_abs,
+           int maxit, int check_every, uint64_t seed, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
+    if (method == 1) return cocg<T>(plan, st, rhs_y, tol_rel, tol_abs, maxit, check_every, eager, autotune, res, err);
+    return tfqmr<T>(plan, st, rhs_y, tol_rel, tol_abs, maxit, check_every, seed, eager, autotune, res, err);
 }
 
-// true residual r = b - A x64 with the FP64 action over the active groups; norms into st.res;
-// optionally r as the next right-hand side
-template <typename T>
-int residual(gfa_plan* plan, SolverState& st, double* y64, T* r_out, int& actions, std::string& err) {
+// true residual b - A x64 with the FP64 action over the active groups: y64 = A x64 (the next refinement's
+// right-hand side is b - y64), norms into st.res
+int residual(gfa_plan* plan, SolverState& st, int& actions, std::string& err) {
     DevicePlan& dev = capi_dev(plan);
     ActiveScope scope(dev, st);
     if (st.nseg == 0) return 0;
     hipStream_t const s = st.stream;
-    if (int r = capi_action(plan, 64, y64, st.x64, s, err)) return r;
+    if (int r = capi_action(plan, 64, st.y64, st.x64, s, err)) return r;
     actions++;
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
-    hipLaunchKernelGGL((k_residual<T>), dim3(st.nseg), dim3(256), 0, s, y64, st.diag, r_out, g);
+    hipLaunchKernelGGL((k_residual<double>), dim3(st.nseg), dim3(256), 0, s, st.y64, st.diag, g);
     hipLaunchKernelGGL(k_col_norm, dim3(st.nCols), dim3(64 * FOLD_P), 0, s, st.res, st.segpart, st.col_seg,
                        st.col_active);
     HIP_TRY(hipGetLastError());
@@ -1307,12 +1334,39 @@ int residual(gfa_plan* plan, SolverState& st, double* y64, T* r_out, int& action
 
 }  // namespace
 
+uint64_t solver_held_bytes(gfa_plan* plan) {
+    SolverState const* st = state_of(plan, false);
+    return st ? uint64_t(st->held_bytes + st->arena_bytes) : 0;
+}
+
+uint64_t solve_bytes(gfa_plan* plan, int fp_bits, int method) {
+    Plan const& h = capi_host(plan);
+    uint64_t const nnzb = h.nnzb, ncol = 64ull * h.nCols;
+    uint64_t nseg = 0;
+    {
+        std::vector<uint64_t> per(h.nCols, 0);
+        for (uint64_t b = 0; b < nnzb; ++b) ++per[h.colindx[b]];
+        for (uint32_t c = 0; c < h.nCols; ++c) nseg += (per[c] + SEG - 1) / SEG;
+    }
+    uint64_t const vec64 = nnzb * NBLK * 8;
+    uint64_t bytes = 2 * vec64;                          // x64, y64
+    bytes += nnzb * (4 + 1 + 4) + nnzb * 128 * 8;        // col_blocks, diag, act_order; the per-block (p, A p)
+    bytes += nseg * (12 + 4) + nseg * 3 * 64 * 8;        // segments, active segments, segment partials
+    bytes += ncol * 8 * (NFIELDS + 2) + uint64_t(h.nCols) * 13;
+    int const m = method == 1 ? 1 : 0;
+    uint64_t arena = uint64_t(fp_bits == 64 ? work_vectors<double>(m) * 2 : work_vectors<float>(m)) * (vec64 / 2);
+    if (fp_bits == 32) arena = std::max(arena, uint64_t(work_vectors<double>(m)) * vec64);   // an FP64 fallback
+    return bytes + arena;
+}
+
 void solver_release(gfa_plan* plan) {
     for (size_t k = 0; k < g_states.size(); ++k) {
         if (g_states[k].first != plan) continue;
         auto* s = g_states[k].second;
         drop_graph(*s);
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
         for (void* p : s->allocations) (void)hipFree(p);
+        if (s->arena) (void)hipFree(s->arena);
         if (s->S_host) (void)hipHostFree(s->S_host);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
@@ -1370,34 +1424,29 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     }
 
     if (o.fp_bits == 64) {
-        if (int r = ensure_work<double>(st, err)) return r;
-        double* y64 = work<double>(st, 0);  // the work vectors are free between TFQMR runs
         TfqmrResult tr;
         if (int r = krylov<double>(o.method, plan, st, nullptr, o.tolerance, nullptr, o.max_iterations, check_every, seed, eager, o.autotune != 0, tr, err)) return r;
         S.iterations += tr.halfsteps; S.actions += tr.actions;
         if (int r = set_active(plan, st, all, err)) return r;
-        if (!st.rhs64 && o.max_refinements > 0) { if (int r = walloc(st, st.rhs64, st.nnzb * NBLK, err)) return r; }
-        if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
+        if (int r = residual(plan, st, S.actions, err)) return r;
         if (int r = fetch_resid()) return r;
         if (o.echo > 0) std::fprintf(stderr, "# gfa_solve fp64: %d half-steps, max residual %.3e\n", tr.halfsteps, max_resid());
-        // FP64 refinement steps if requested and needed: solve A delta = r, x += delta
+        // FP64 refinement steps if requested and needed: solve A delta = b - A x, x += delta
         for (int k = 0; k < o.max_refinements && max_resid() > o.tolerance; ++k) {
             auto const act = unconverged();
             if (int r = set_active(plan, st, act, err)) return r;
             TfqmrResult t2;
-            if (int r = krylov<double>(o.method, plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every, seed + k + 1, eager, false, t2, err)) return r;
+            if (int r = krylov<double>(o.method, plan, st, st.y64, 1e-30, st.tol_abs, o.max_iterations, check_every, seed + k + 1, eager, false, t2, err)) return r;
             S.iterations += t2.halfsteps; S.actions += t2.actions; S.refinements++;
             if (int r = set_active(plan, st, act, err)) return r;
-            if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
+            if (int r = residual(plan, st, S.actions, err)) return r;
             if (int r = fetch_resid()) return r;
         }
     } else {
-        // FP32 tfQMR inside FP64 refinement, per column group: a group stays in FP32 while each FP32 solve
+        // FP32 inner solves inside FP64 refinement, per column group: a group stays in FP32 while each FP32 solve
         // reduces its true (FP64) residual at least tenfold; a group whose FP32 solve gains less (the FP32
-        // recurrences of an ill-conditioned instance stall) is solved again in FP64 from zero.
-        if (int r = ensure_work<float>(st, err)) return r;
-        if (!st.rhs32) { if (int r = walloc(st, st.rhs32, st.nnzb * NBLK, err)) return r; }
-        double* y64_32 = reinterpret_cast<double*>(work<float>(st, 0));  // spans work vectors 0 and 1
+        // recurrences of an ill-conditioned instance stall) is solved again in FP64 from zero.  The FP64 work
+        // vectors take the arena of the FP32 ones (the two never run at the same time).
         double const floor_rel = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;  // gain asked of one FP32 solve
         int const budget32 = o.fp32_max_iterations > 0 ? o.fp32_max_iterations : o.max_iterations;
         std::vector<uint8_t> prec64(st.nCols, 0), done(st.nCols, 0);
@@ -1415,12 +1464,12 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             if (any32) {
                 if (int r = set_active(plan, st, a32, err)) return r;
                 TfqmrResult tr;
-                if (int r = krylov<float>(o.method, plan, st, first32 ? nullptr : st.rhs32, floor_rel, st.tol_abs, budget32,
+                if (int r = krylov<float>(o.method, plan, st, first32 ? nullptr : st.y64, floor_rel, st.tol_abs, budget32,
                                          check_every, seed + k, eager, o.autotune != 0, tr, err)) return r;
                 first32 = false;
                 S.iterations += tr.halfsteps; S.actions += tr.actions;
                 if (int r = set_active(plan, st, a32, err)) return r;
-                if (int r = residual<float>(plan, st, y64_32, st.rhs32, S.actions, err)) return r;
+                if (int r = residual(plan, st, S.actions, err)) return r;
                 if (int r = fetch_resid()) return r;
                 double mx = 0;
                 for (uint32_t c = 0; c < st.nCols; ++c) {
@@ -1437,27 +1486,24 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             bool any_fresh = false;
             for (uint32_t c = 0; c < st.nCols; ++c) any_fresh |= fresh64[c] != 0;
             if (any64 || any_fresh) {
-                if (int r = ensure_work<double>(st, err)) return r;
-                if (!st.rhs64) { if (int r = walloc(st, st.rhs64, st.nnzb * NBLK, err)) return r; }
-                double* y64 = work<double>(st, 0);
                 if (any_fresh) {
                     // groups leaving FP32 start over in FP64 from x = 0 (continuing from the stalled FP32
-                    // iterate converges more slowly than a fresh FP64 solve): rhs = b - A 0 = b
+                    // iterate converges more slowly than a fresh FP64 solve): y = A 0 = 0, rhs = b
                     if (int r = set_active(plan, st, fresh64, err)) return r;
                     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
                     hipLaunchKernelGGL((k_zero_seg<double>), dim3(st.nseg), dim3(256), 0, s, st.x64, g);
+                    hipLaunchKernelGGL((k_zero_seg<double>), dim3(st.nseg), dim3(256), 0, s, st.y64, g);
                     HIP_TRY(hipGetLastError());
-                    if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
                 }
                 std::vector<uint8_t> b64(st.nCols, 0);
                 for (uint32_t c = 0; c < st.nCols; ++c) b64[c] = (!done[c] && prec64[c]) ? 1 : 0;
                 if (int r = set_active(plan, st, b64, err)) return r;
                 TfqmrResult t2;
-                if (int r = krylov<double>(o.method, plan, st, st.rhs64, 1e-30, st.tol_abs, o.max_iterations, check_every,
+                if (int r = krylov<double>(o.method, plan, st, st.y64, 1e-30, st.tol_abs, o.max_iterations, check_every,
                                           seed + 1000 + k, eager, o.autotune != 0, t2, err)) return r;
                 S.iterations += t2.halfsteps; S.actions += t2.actions;
                 if (int r = set_active(plan, st, b64, err)) return r;
-                if (int r = residual<double>(plan, st, y64, st.rhs64, S.actions, err)) return r;
+                if (int r = residual(plan, st, S.actions, err)) return r;
                 if (int r = fetch_resid()) return r;
                 double mx = 0;
                 for (uint32_t c = 0; c < st.nCols; ++c) {

@@ -11,7 +11,13 @@
 namespace gfa {
 
 int solve(gfa_plan* plan, gfa_solve_opts const& opts, double* rho, gfa_solve_stats* stats, std::string& err);
+// frees the solver state of a plan (work vectors, solution, captured graph); the next solve allocates again
 void solver_release(gfa_plan* plan);
+// device bytes of the solver state of a solve with these options at its largest (FP32 solves: including the FP64
+// work vectors of a column group that falls back to FP64)
+uint64_t solve_bytes(gfa_plan* plan, int fp_bits, int method);
+// device bytes the solver state of a plan holds now
+uint64_t solver_held_bytes(gfa_plan* plan);
 
 // provided by gfa_capi.cpp
 int capi_action(gfa_plan* p, int fp_bits, void* y, void const* x, hipStream_t stream, std::string& err,

@@ -25,7 +25,7 @@ EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
              "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_action_part", "gfa_gradient",
-             "gfa_solve", "gfa_atom_density_matrices", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density",
+             "gfa_solve", "gfa_solver_release", "gfa_solve_memory", "gfa_plan_prepare", "gfa_atom_density_matrices", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density",
              "gfa_last_error"]
 
 
@@ -46,7 +46,8 @@ class Sizes(C.Structure):
                 ("nAtomImages", C.c_uint32), ("natomcoeffs", C.c_uint64), ("ncoeffs", C.c_uint64),
                 ("nlists", C.c_uint32 * 3), ("fd_range", C.c_int32 * 3), ("nlist_entries", C.c_uint64 * 3),
                 ("sho_add_nnz", C.c_uint64), ("sho_prj_nnz", C.c_uint64), ("sho_sum_nnz", C.c_uint64),
-                ("bc", C.c_int8 * 3), ("reserved", C.c_int8), ("nchunks", C.c_uint32), ("device_bytes", C.c_uint64)]
+                ("bc", C.c_int8 * 3), ("reserved", C.c_int8), ("nchunks", C.c_uint32), ("device_bytes", C.c_uint64),
+                ("solver_bytes", C.c_uint64)]
 
 
 class SolveOpts(C.Structure):
@@ -110,6 +111,9 @@ def lib():
         L.gfa_gradient.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), vp]
         L.gfa_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_double), C.POINTER(SolveStats)]
+        L.gfa_solver_release.argtypes = [vp]
+        L.gfa_solve_memory.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.gfa_plan_prepare.argtypes = [vp, C.c_int]
         dp = C.POINTER(C.c_double)
         L.gfa_atom_density_matrices.argtypes = [vp, vp, C.c_double, dp, vp]
         L.gfa_gauss_legendre.argtypes = [C.c_int32, dp, dp]
@@ -355,6 +359,25 @@ class GreenPlan:
         v = [C.c_int() for _ in range(4)]
         _check(lib().gfa_get_tile_shape(self.h, fp_bits, *[C.byref(t) for t in v]))
         return tuple(t.value for t in v)
+
+    def prepare(self, fp_bits):
+        """Upload the plan and allocate the action's workspace of fp_bits (later actions allocate nothing)."""
+        _check(lib().gfa_plan_prepare(self.h, fp_bits))
+        self.refresh_sizes()
+
+    def release_solver(self):
+        """Free the solver state (solution, work vectors, captured graph); the plan stays usable."""
+        _check(lib().gfa_solver_release(self.h))
+        self.refresh_sizes()
+
+    def solve_memory(self, fp_bits=64, method="tfqmr"):
+        """(solver bytes at the solve's largest, plan device bytes) of a solve with these options."""
+        o = SolveOpts()
+        o.fp_bits = fp_bits
+        o.method = {"tfqmr": 0, "cocg": 1}[method]
+        sb, pb = C.c_uint64(), C.c_uint64()
+        _check(lib().gfa_solve_memory(self.h, C.byref(o), C.byref(sb), C.byref(pb)))
+        return sb.value, pb.value
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
               shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0, method="tfqmr"):

@@ -6,12 +6,19 @@ E = -1 Ha to E_F = 0 Ha (Im E 0.005 .. 0.3 Ha) x 4 seeded k-points, truncation r
 {6, 8, 10, 12} Bohr per instance; one source block (64 columns) per instance; 8th-order kinetic stencil.
 The cell is wrapped by every radius (2 r <= 32 Bohr), so the k-point leaves the operator unchanged, as in the
 reference.  A step solves the 32 instances of one k-point as one batched plan (column groups = instances)
-with FP32 COCG (the operator is complex symmetric: k is inert under Wrap) inside FP64 refinement; every column
-is verified with the FP64 action and only columns whose true residual meets 1e-9 are counted.
+with FP32 COCG (the operator is complex symmetric: k is inert under Wrap) inside FP64 refinement; steps cycle
+over the 4 k-points, so every 4 steps cover the 128 instances.  Every column is verified with the FP64 action
+and only columns whose true residual meets 1e-9 are counted.
 
-Beside it: the plain FP64 path on the same 32 instances (one plan and tfQMR solve per instance, as the reference's
-contour loop calls its solver per energy), the FP64 batched tfQMR and COCG solves, the local kernel's roofline from HIP events
-on its launch stream, and the reference's own CPU action (oracle/_ref, single thread) on a bounded sample.
+Each step's plan is built, uploaded and given the tile shapes the warm-up tuned before its timer starts; its
+solver state is released after the step, outside the timer.  The value is the converged columns of the timed
+steps over the sum of their solve times.
+
+Beside it: the FP64 solution of every batch (FP64 COCG, batched) and the mixed solution's error against it; on a
+sample of 32 instances spread over the 4 batches (every energy once) the plain FP64 path (one plan and FP64 tfQMR
+solve per instance, as the reference's contour loop calls its solver per energy); on batch 0 the FP64 tfQMR and
+mixed tfQMR batched solves; the local kernel's roofline from HIP events on its launch stream; and the oracle's
+CPU TFQMR (single thread) on a bounded sample.
 """
 import argparse
 import json
@@ -31,27 +38,31 @@ FP32_SHAPES = [(0, 16, 1), (0, 16, 2), (0, 16, 4), (0, 32, 1), (0, 32, 2), (0, 3
 # WRITE_SIZE of --mode action, scaled by a plain copy's counts); the command that made them is in the file
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "local_fp32_contour32_traffic.json")
 SOURCE = (16, 16, 16)
+NBATCH = 4
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1, help="batches of one k-point's contour instances")
+    ap.add_argument("--steps", type=int, default=4, help="batches of one k-point's contour instances (cycling k)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=32, help="instances per batch (<= 32)")
     ap.add_argument("--maxit", type=int, default=8000)
     ap.add_argument("--fp32-maxit", type=int, default=3000, help="operator applications allowed to one FP32 solve")
     ap.add_argument("--method", choices=["cocg", "tfqmr"], default="cocg",
-                    help="inner Krylov method of the mixed solve (the FP64 comparison always runs tfQMR)")
+                    help="inner Krylov method of the mixed solve")
     ap.add_argument("--inner-tol", type=float, default=0.0,
                     help="residual reduction asked of one FP32 solve (0: COCG with residual replacement goes to the "
                          "final tolerance in one solve; tfQMR uses 1e-4)")
     ap.add_argument("--action-reps", type=int, default=10)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-actions", type=int, default=4)
-    ap.add_argument("--mode", choices=["bench", "action", "contour"], default="bench",
-                    help="action: only apply the action a few times (for profiler runs)")
+    ap.add_argument("--cpu-halfsteps", type=int, default=10, help="half-steps of the timed CPU TFQMR sample")
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="wall seconds after which the optional batch-0 comparisons are skipped")
+    ap.add_argument("--mode", choices=["bench", "action", "contour", "batch128"], default="bench",
+                    help="action: only apply the action a few times (for profiler runs); batch128: the 128 "
+                         "instances in one plan")
     return ap.parse_args()
 
 
@@ -60,13 +71,14 @@ def log(msg):
 
 
 def batch(instances, k, n):
-    return instances[32 * (k % 4): 32 * (k % 4) + n]
+    b = k % NBATCH
+    return instances[32 * b: 32 * b + n]
 
 
 def batch_plan(G, sysm, inst):
-    import numpy as np
     p = G.GreenPlan(sysm, r_trunc=max(d["radius"] for d in inst), fd_range=(4, 4, 4), sources=[SOURCE] * len(inst),
                     source_radii=[d["radius"] for d in inst])
+    import numpy as np
     p.set_inputs(np.array([d["E"] for d in inst]), inst[0]["k"])
     return p
 
@@ -89,6 +101,13 @@ def time_action(plan, fp, part, reps, stream):
     return ev[0].elapsed_time(ev[1]) / 1e3 / reps
 
 
+def rel_error(A, B):
+    """max |A - B| / max |B| over the whole batch, and per column group (64 columns of one instance)."""
+    import torch
+    d = (A - B).abs().amax(dim=(1, 2, 3))
+    return float((A - B).abs().max() / B.abs().max()), d
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -102,6 +121,7 @@ def main():
     from angstromcube_amd import gfa as G
     from angstromcube_amd import synthetic as S
 
+    t_begin = time.time()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
@@ -112,25 +132,8 @@ def main():
     log(f"# system built in {time.time() - t0:.1f} s")
 
     if args.mode == "action":
-        p = batch_plan(G, sysm, batch(instances, 0, n))
-        for part in (3, 2):
-            t = time_action(p, 64, part, args.action_reps, stream)
-            log(f"# action fp64 part {part}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
-        t = time_action(p, 32, 3, args.action_reps, stream)
-        log(f"# action fp32 part 3: {t * 1e3:.3f} ms for {p.nnzb} blocks")
-        for shape in FP32_SHAPES:   # each FP32 local-kernel shape, in this order (per-dispatch counters follow it)
-            p.set_tile_shape(32, *shape)
-            t = time_action(p, 32, 2, args.action_reps, stream)
-            log(f"# local fp32 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
-        # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
-        x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
-        y = torch.empty_like(x)
-        for _ in range(2):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        log(f"# copy calibration: {x.numel() * 8} bytes read and written per copy")
+        action_mode(G, sysm, instances, n, args, stream, dev)
         return
-
     if args.mode == "contour":
         contour_mode(G, sysm, args)
         return
@@ -138,80 +141,135 @@ def main():
     inner = args.inner_tol if args.inner_tol > 0 else (1e-30 if args.method == "cocg" else 1e-4)
     opts = dict(tolerance=1e-9, max_iterations=args.maxit, max_refinements=6, inner_tolerance=inner,
                 fp32_max_iterations=args.fp32_maxit, method=args.method)
-    # ---- warmup (also tunes the tile shapes of both precisions) ----
+
+    if args.mode == "batch128":
+        batch128_mode(G, sysm, instances, opts)
+        return
+
+    # ---- warmup (the first one also tunes the tile shapes of both precisions) ----
+    shapes = None
     for w in range(args.warmup):
         p = batch_plan(G, sysm, batch(instances, w, n))
+        if shapes:
+            for fp in (32, 64):
+                p.set_tile_shape(fp, *shapes[fp])
         _, st = p.solve(fp_bits=32, autotune=True, **opts)
-        log(f"# warmup {w}: {st['converged_columns']}/{st['total_columns']} columns, {st['seconds']:.2f} s")
+        if shapes is None:
+            shapes = {fp: p.tile_shape(fp)[:3] for fp in (32, 64)}
+        log(f"# warmup {w}: {st['converged_columns']}/{st['total_columns']} columns, {st['seconds']:.2f} s, "
+            f"tile shapes {shapes}")
         p.close()
 
-    # ---- the measured path: batched mixed-precision solves ----
-    plans = [batch_plan(G, sysm, batch(instances, k, n)) for k in range(args.steps)]
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    stats = []
-    for k in range(args.steps):
-        _, st = plans[k].solve(fp_bits=32, autotune=True, **opts)
-        stats.append(st)
-    torch.cuda.synchronize()
-    t_mixed = time.perf_counter() - t_start
-    cols_mixed = sum(s["converged_columns"] for s in stats)
+    # ---- the measured path: batched mixed-precision solves, one plan per step ----
+    t_mixed = 0.0
+    cols_mixed = 0
+    per_batch = {}
+    errors = {}
+    fp64_cocg = {}
     per_inst = []
-    for k, st in enumerate(stats):
-        hs = st["column_halfsteps"].reshape(-1, 64).max(axis=1)
-        log(f"# step {k}: {st['converged_columns']}/{st['total_columns']} columns in {st['seconds']:.2f} s, "
-            f"{st['iterations']} half-steps, max residual {st['max_residual']:.2e}")
-        if k == 0:
-            per_inst = [int(h) for h in hs]
-    nnzb = plans[0].nnzb
-    for p in plans[1:]:
-        p.close()
-    plan = plans[0]
-    first = batch(instances, 0, n)
-
-    # ---- FP64: the same instances batched, and one at a time (the plain drop-in path) ----
-    fp64 = {}
-    if not args.no_fp64:
+    for k in range(args.steps):
+        b = k % NBATCH
+        inst = batch(instances, k, n)
+        p = batch_plan(G, sysm, inst)   # untimed: host plan, device upload, workspaces, pinned tile shapes
+        if shapes:
+            for fp in (32, 64):
+                p.set_tile_shape(fp, *shapes[fp])
+        p.prepare(32)
+        p.prepare(64)
+        keep = (b not in errors) and not args.no_fp64
+        X32 = torch.empty((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev) if keep else None
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        _, s64 = plan.solve(fp_bits=64, autotune=True, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2)
-        t_b64 = time.perf_counter() - t1
-        log(f"# fp64 batched: {s64['converged_columns']} columns in {t_b64:.2f} s")
-        fp64["batched_columns_per_s"] = round(s64["converged_columns"] / t_b64, 3)
-        fp64["batched_half_steps"] = [int(h) for h in s64["column_halfsteps"].reshape(-1, 64).max(axis=1)]
-        if args.method == "cocg":   # the same batch in FP64 with the mixed path's inner method
+        _, st = p.solve(fp_bits=32, autotune=True, X=X32, **opts)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        t_mixed += dt
+        cols_mixed += st["converged_columns"]
+        per_batch.setdefault(b, []).append(st["converged_columns"] / dt)
+        log(f"# step {k} (k-point {b}, {p.nnzb} blocks): {st['converged_columns']}/{st['total_columns']} columns "
+            f"in {dt:.2f} s, {st['iterations']} applications, max residual {st['max_residual']:.2e}, "
+            f"solver {p.sizes.solver_bytes / 1e9:.1f} GB")
+        if k == 0:
+            per_inst = [int(h) for h in st["column_halfsteps"].reshape(-1, 64).max(axis=1)]
+        if keep:
+            # the FP64 solution of the same batch (FP64 COCG, batched), untimed for the mixed rate
+            p.release_solver()
+            X64 = torch.empty_like(X32)
             torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            _, sc = plan.solve(fp_bits=64, autotune=True, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2,
-                               method="cocg")
-            t_c = time.perf_counter() - t1
-            log(f"# fp64 cocg batched: {sc['converged_columns']} columns in {t_c:.2f} s")
-            fp64["cocg_batched_columns_per_s"] = round(sc["converged_columns"] / t_c, 3)
-            # the mixed path with the reference's method (FP32 tfQMR inside FP64 refinement), same batch
+            t2 = time.perf_counter()
+            _, s64 = p.solve(fp_bits=64, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2,
+                             method="cocg" if args.method == "cocg" else "tfqmr", X=X64)
             torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            _, sm = plan.solve(fp_bits=32, autotune=True, tolerance=1e-9, max_iterations=args.maxit, max_refinements=6,
-                               inner_tolerance=1e-4, fp32_max_iterations=1000, method="tfqmr")
-            t_m = time.perf_counter() - t1
-            log(f"# mixed tfqmr batched: {sm['converged_columns']} columns in {t_m:.2f} s")
-            fp64["mixed_tfqmr_batched_columns_per_s"] = round(sm["converged_columns"] / t_m, 3)
-        cols, t_seq = 0, 0.0
-        for d in first:   # plan built untimed (as the batched plans), device upload and solve timed
+            t64 = time.perf_counter() - t2
+            fp64_cocg[b] = s64["converged_columns"] / t64
+            e, per_group = rel_error(X32, X64)
+            errors[b] = e
+            log(f"#   fp64 {'cocg' if args.method == 'cocg' else 'tfqmr'} batched: {s64['converged_columns']} "
+                f"columns in {t64:.2f} s; |X_mixed - X_fp64| / |X_fp64| = {e:.2e}")
+            del X32, X64, per_group
+        p.close()
+    value = cols_mixed / t_mixed
+
+    # ---- FP64 comparisons ----
+    fp64 = {}
+    first = batch(instances, 0, n)
+    if not args.no_fp64:
+        fp64["batched_columns_per_s_by_batch"] = {str(b): round(v, 3) for b, v in sorted(fp64_cocg.items())}
+        # the plain FP64 path: one plan and FP64 tfQMR solve per instance, on 32 instances spread over the batches
+        sample = [instances[32 * b + i] for b in range(NBATCH) for i in range(b, 32, NBATCH)]
+        cols, t_seq, hs_seq, nb_seq = 0, 0.0, [], []
+        for d in sample:
             p1 = batch_plan(G, sysm, [d])
+            if shapes:
+                p1.set_tile_shape(64, *shapes[64])
+            p1.prepare(64)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             _, s1 = p1.solve(fp_bits=64, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2)
+            torch.cuda.synchronize()
             t_seq += time.perf_counter() - t1
             cols += s1["converged_columns"]
+            hs_seq.append(int(s1["column_halfsteps"].max()))
+            nb_seq.append(int(p1.nnzb))
             p1.close()
-        log(f"# fp64 one instance at a time: {cols} columns in {t_seq:.2f} s")
+        log(f"# fp64 one instance at a time ({len(sample)} instances, every energy once): {cols} columns in "
+            f"{t_seq:.2f} s")
         fp64.update(columns_per_s=round(cols / t_seq, 3), seconds=round(t_seq, 3), converged_columns=cols,
-                    actions=int(s64["actions"]))
+                    sample="32 instances: from batch b the energies b, b+4, ..., b+28")
+        # batch 0 with the other methods, while the wall-clock budget allows
+        if time.time() - t_begin < args.budget_s:
+            p = batch_plan(G, sysm, first)
+            for fp in (32, 64):
+                if shapes:
+                    p.set_tile_shape(fp, *shapes[fp])
+            p.prepare(32)
+            p.prepare(64)
+            for name, kw in (("tfqmr_batched_columns_per_s",
+                              dict(fp_bits=64, tolerance=1e-9, max_iterations=args.maxit, max_refinements=2)),
+                             ("mixed_tfqmr_batched_columns_per_s",
+                              dict(fp_bits=32, tolerance=1e-9, max_iterations=args.maxit, max_refinements=6,
+                                   inner_tolerance=1e-4, fp32_max_iterations=1000, method="tfqmr"))):
+                if time.time() - t_begin > args.budget_s:
+                    break
+                p.release_solver()
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                _, sx = p.solve(**kw)
+                torch.cuda.synchronize()
+                tx = time.perf_counter() - t1
+                fp64[name + "_batch0"] = round(sx["converged_columns"] / tx, 3)
+                log(f"# batch 0 {name}: {sx['converged_columns']} columns in {tx:.2f} s")
+            p.close()
 
     # ---- roofline of the local kernel (potential + kinetic + SHO add-back), HIP events on its stream ----
+    plan = batch_plan(G, sysm, first)
+    if shapes:
+        for fp in (32, 64):
+            plan.set_tile_shape(fp, *shapes[fp])
+    nnzb = plan.nnzb
     t_loc64 = time_action(plan, 64, 2, args.action_reps, stream)
     t_loc32 = time_action(plan, 32, 2, args.action_reps, stream)
-    shape32 = plan.tile_shape(32)[:3]   # the FP32 local kernel the mixed solve ran (autotuned)
+    shape32 = plan.tile_shape(32)[:3]   # the FP32 local kernel the mixed solve ran
     t_act64 = time_action(plan, 64, 3, args.action_reps, stream)
     t_act32 = time_action(plan, 32, 3, args.action_reps, stream)
     nrows = plan.nRows
@@ -219,10 +277,10 @@ def main():
     alg32 = nnzb * 2 * BLOCK_BYTES[32] + nrows * 64 * 8
     plan.close()
 
-    value = cols_mixed / t_mixed
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sysm, first, per_inst, fp64.get("batched_half_steps"), args)
+        cpu = cpu_baseline(G, sysm, sample if not args.no_fp64 else first,
+                           hs_seq if not args.no_fp64 else per_inst, args)
 
     out = {
         "metric": "Gf columns solved/s at resid 1e-9 (mixed+batched vs FP64); action % HBM peak",
@@ -233,28 +291,77 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * t_mixed / args.steps, 3),
         "higher_is_better": True,
-        "scaling": "weak",
+        "scaling": "none (one GPU)",
         "vs_baseline": None,
         "dtype": f"c32 {args.method} + f64 refinement (per-group FP64 fallback)",
         "data": "synthetic (seeded Gaussian potential, random atom matrices, Gauss-Legendre contour)",
         "config": {"workload": "batched contour: 32^3-block cell, 128 atoms numax 5, 32 energies -1..0 Ha "
-                               "(Im 0.005..0.3) x 4 k-points, radii {6,8,10,12} Bohr, one source block each",
-                   "instances_per_step": n, "columns_per_step": 64 * n, "nnzb_per_step": int(nnzb)},
-        "mixed_half_steps_per_instance": per_inst,
+                               "(Im 0.005..0.3) x 4 k-points, radii {6,8,10,12} Bohr, one source block each; "
+                               "a step = the 32 instances of one k-point in one plan",
+                   "instances_per_step": n, "columns_per_step": 64 * n},
+        "columns_per_s_by_batch": {str(b): round(float(np.mean(v)), 3) for b, v in sorted(per_batch.items())},
+        "mixed_vs_fp64_max_rel_error": max(errors.values()) if errors else None,
+        "mixed_vs_fp64_rel_error_by_batch": {str(b): v for b, v in sorted(errors.items())} or None,
+        "mixed_half_steps_per_instance_batch0": per_inst,
         "fp64": fp64 or None,
         "mixed_over_fp64": round(value / fp64["columns_per_s"], 3) if fp64 else None,
-        "mixed_over_fp64_batched": round(value / fp64["batched_columns_per_s"], 3) if fp64 else None,
         "action": {"fp64_ms": round(t_act64 * 1e3, 3), "fp32_ms": round(t_act32 * 1e3, 3),
                    "local_fp64_ms": round(t_loc64 * 1e3, 3), "local_fp32_ms": round(t_loc32 * 1e3, 3),
-                   "fp64_GBs": round(alg64 / t_act64 / 1e9, 1), "fp32_GBs": round(alg32 / t_act32 / 1e9, 1)},
+                   "fp64_GBs": round(alg64 / t_act64 / 1e9, 1), "fp32_GBs": round(alg32 / t_act32 / 1e9, 1),
+                   "nnzb_batch0": int(nnzb),
+                   "local_fp64_frac": round(alg64 / t_loc64 / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(alg32 / t_loc32 / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg32 / t_loc32 / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": traffic_of(shape32, nnzb),
                      "kernel": f"FP32 local kernel, tile shape {shape32} ({'k_local' if shape32[0] == 0 else 'k_plane'}: "
-                               "potential + 8th-order kinetic + SHO add-back), the one the mixed solve ran; "
-                               "algorithmic bytes per launch = nnzb * 2 * 32 KiB + nRows * 512 B"},
+                               "potential + 8th-order kinetic + SHO add-back), the one the mixed solve ran, on the "
+                               "batch-0 plan; algorithmic bytes per launch = nnzb * 2 * 32 KiB + nRows * 512 B"},
         "cpu_baseline": cpu,
+        "wall_s": round(time.time() - t_begin, 1),
     }
+    print(json.dumps(out))
+
+
+def action_mode(G, sysm, instances, n, args, stream, dev):
+    import torch
+    p = batch_plan(G, sysm, batch(instances, 0, n))
+    for part in (3, 2):
+        t = time_action(p, 64, part, args.action_reps, stream)
+        log(f"# action fp64 part {part}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+    t = time_action(p, 32, 3, args.action_reps, stream)
+    log(f"# action fp32 part 3: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+    for shape in FP32_SHAPES:   # each FP32 local-kernel shape, in this order (per-dispatch counters follow it)
+        p.set_tile_shape(32, *shape)
+        t = time_action(p, 32, 2, args.action_reps, stream)
+        log(f"# local fp32 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+    # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
+    x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"# copy calibration: {x.numel() * 8} bytes read and written per copy")
+
+
+def batch128_mode(G, sysm, instances, opts):
+    """The 128 instances (4 k-points x 32 energies) in one plan and one solve (the k-point is inert under Wrap)."""
+    import torch
+    p = batch_plan(G, sysm, instances)
+    sb, pb = p.solve_memory(32, opts["method"])
+    log(f"# 128-instance plan: {p.nnzb} blocks, solver up to {sb / 1e9:.1f} GB, plan {pb / 1e9:.1f} GB")
+    p.prepare(32)
+    p.prepare(64)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st = p.solve(fp_bits=32, autotune=True, **opts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    out = {"mode": "batch128", "instances": len(instances), "nnzb": int(p.nnzb),
+           "converged_columns": int(st["converged_columns"]), "total_columns": int(st["total_columns"]),
+           "seconds": round(dt, 3), "columns_per_s": round(st["converged_columns"] / dt, 3),
+           "max_residual": st["max_residual"], "solver_bytes": int(p.sizes.solver_bytes),
+           "plan_bytes": int(p.sizes.device_bytes), "applications": int(st["iterations"])}
+    p.close()
     print(json.dumps(out))
 
 
@@ -307,59 +414,37 @@ def traffic_of(shape, nnzb):
     return int(e["fetch_bytes"] + e["write_bytes"])
 
 
-def cpu_baseline(sysm, first, half_steps_mixed, half_steps_fp64, args):
-    """The reference's own CPU action (built from its sources into oracle/_ref; the oracle's C restatement if
-    that build is absent), single thread, on one radius-8 instance; scaled by block count to every instance
-    and by the FP64 solve's actions per instance to converged columns per second (vector updates of the CPU
-    solve not counted, so this is an upper bound of the CPU rate)."""
-    import numpy as np
-    from angstromcube_amd import gfa as G
-    from oracle import refgreen as R
+def cpu_baseline(G, sysm, sample, half_steps, args):
+    """The oracle's CPU TFQMR (a C restatement of the reference's action with the same TFQMR as the device: actions,
+    vector updates and inner products), single thread, timed for --cpu-halfsteps half-steps on the radius-6
+    instance of the sample; the time per block and half-step is scaled to the sample's instances by their block
+    counts and the half-steps their plain FP64 GPU solves took (the same algorithm), giving columns per second."""
+    from oracle import oracle as O
+    small = min(sample, key=lambda d: d["radius"])
+    p = G.GreenPlan(sysm, r_trunc=small["radius"], fd_range=(4, 4, 4), sources=[SOURCE])
+    p.set_inputs(small["E"], small["k"])
+    o = O.Oracle(p, small["E"], small["k"])
+    nb = p.nnzb
+    nhs = max(2, args.cpu_halfsteps)
+    t = time.perf_counter()
+    _, _, m = o.tfqmr(tol=1e-30, maxit=nhs)
+    t_run = time.perf_counter() - t
+    # the run applies the operator once per half-step, once to start and once for the final residual
+    per_block_hs = t_run / ((m + 2) * nb)
+    p.close()
     nblocks = {}
-    for d in first:
+    for d in sample:
         r = d["radius"]
         if r not in nblocks:
-            p = G.GreenPlan(sysm, r_trunc=r, fd_range=(4, 4, 4), sources=[SOURCE])
-            nblocks[r] = p.nnzb
-            p.close()
-    sample = next(d for d in first if d["radius"] == 8.0) if any(d["radius"] == 8.0 for d in first) else first[0]
-    kind = "reference"
-    x = np.random.default_rng(3).standard_normal
-    try:
-        if not R.available():
-            raise RuntimeError("no reference build")
-        rp = R.RefPlan(sysm, r_trunc=sample["radius"], fd_range=(4, 4, 4), nsrc=(1, 1, 1))
-        rp.set_inputs(sample["E"])
-        nb = rp.nnzb
-        xv = x((nb, 2, 64, 64))
-        t = time.perf_counter()
-        for _ in range(args.cpu_actions):
-            rp.action(xv)
-        t_action = (time.perf_counter() - t) / args.cpu_actions
-    except Exception as e:
-        log(f"# cpu baseline falls back to the oracle: {e}")
-        kind = "port"
-        from oracle import oracle as O
-        p = G.GreenPlan(sysm, r_trunc=sample["radius"], fd_range=(4, 4, 4), sources=[SOURCE])
-        o = O.Oracle(p, sample["E"])
-        nb = p.nnzb
-        xv = x((nb, 2, 64, 64))
-        t = time.perf_counter()
-        for _ in range(args.cpu_actions):
-            o.action(xv)
-        t_action = (time.perf_counter() - t) / args.cpu_actions
-        p.close()
-    per_block = t_action / nb
-    hs = half_steps_fp64 or half_steps_mixed
-    if not hs:
-        return {"value": None, "unit": "Gf columns/s", "cores": 1, "kind": kind,
-                "sample": f"{args.cpu_actions} CPU actions of {t_action:.2f} s; no iteration counts"}
-    t_cpu = sum(per_block * nblocks[d["radius"]] * (h + 2) for d, h in zip(first, hs))
-    return {"value": round(64.0 * len(first) / t_cpu, 6), "unit": "Gf columns/s", "cores": 1, "kind": kind,
-            "sample": f"{args.cpu_actions} single-thread CPU actions on the radius-{sample['radius']:g} instance "
-                      f"({nb} blocks, {t_action:.2f} s each), scaled by block count to the {len(first)} instances "
-                      f"and by the FP64 GPU solve's actions per instance; vector updates not counted; "
-                      f"host {platform.processor() or platform.machine()}"}
+            q = G.GreenPlan(sysm, r_trunc=r, fd_range=(4, 4, 4), sources=[SOURCE])
+            nblocks[r] = q.nnzb
+            q.close()
+    t_cpu = sum(per_block_hs * nblocks[d["radius"]] * (h + 2) for d, h in zip(sample, half_steps))
+    return {"value": round(64.0 * len(sample) / t_cpu, 6), "unit": "Gf columns/s", "cores": 1, "kind": "port",
+            "sample": f"oracle CPU TFQMR, {m} half-steps on the radius-{small['radius']:g} instance ({nb} blocks) in "
+                      f"{t_run:.1f} s (actions, vector updates and inner products), scaled by block count and by "
+                      f"the FP64 GPU solves' half-steps to the {len(sample)} instances of the plain FP64 sample; "
+                      f"host {platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs"}
 
 
 if __name__ == "__main__":

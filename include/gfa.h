@@ -64,7 +64,8 @@ typedef struct {
     int8_t bc[3];               /* boundary conditions after wrapping (5 = wrap)                   */
     int8_t reserved;
     uint32_t nchunks;           /* projection work items                                           */
-    uint64_t device_bytes;      /* device memory currently held by the plan                        */
+    uint64_t device_bytes;      /* device memory currently held by the plan's data and action workspace */
+    uint64_t solver_bytes;      /* device memory currently held by the plan's solver state (gfa_solver_release frees it) */
 } gfa_sizes;
 
 /* arrays for gfa_plan_export (host copies, for verification against the reference) */
@@ -196,8 +197,20 @@ typedef struct {
 } gfa_solve_stats;
 
 /* rho[nCols][64][2]: Green function diagonal G_ii / pi (re, im) as include/green_action.hxx:226-238;
- * may be NULL. */
+ * may be NULL.  The solver state (the FP64 solution, A x, the inner solver's work vectors) stays allocated with the
+ * plan after the call, so that a following solve of the same plan allocates nothing; gfa_solver_release frees it. */
 int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve_stats* stats);
+
+/* ---- device memory of a solve.  gfa_solver_release frees the solver state of a plan (the reference frees the
+ *      tfQMRgpu buffer with its action_t, include/green_action.hxx:118-123); the plan itself stays usable.
+ *      gfa_solve_memory gives the device bytes of the solver state a solve with opts (fp_bits, method) holds at its
+ *      largest -- for fp_bits 32 including the FP64 work vectors of a column group that falls back to FP64 -- and
+ *      the bytes of the plan's device data (an estimate before the plan is first used on the device).
+ *      gfa_plan_prepare uploads the plan and allocates the action's workspace of fp_bits, so that later actions
+ *      of that precision allocate nothing (for timing, graph capture and gfa_action_noalloc). ---- */
+int gfa_solver_release(gfa_plan* plan);
+int gfa_solve_memory(gfa_plan* plan, const gfa_solve_opts* opts, uint64_t* solver_bytes, uint64_t* plan_bytes);
+int gfa_plan_prepare(gfa_plan* plan, int fp_bits);
 
 /* ---- adjoint gradient of L = Im sum_c w_c Tr[G_c], the trace over the diagonal block of column group c ----
  *  With dG = -A^-1 dA G:  dL = -Im sum_c w_c <lambda_c, dA G_c>,  lambda_c = (H - E_c S)^-H B_c.

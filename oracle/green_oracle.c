@@ -413,7 +413,12 @@ double oracle_shadow(uint64_t seed, uint64_t idx) {
     return (double)((int32_t)h >> 8) * (1.0 / 8388608.0);
 }
 
-/* ---- TFQMR, Saad Algorithm 7.8, per column; the same scalar formulas as the device solver ---- */
+/* ---- TFQMR, Saad Algorithm 7.8, per column; the same scalar formulas as the device solver ----
+ * The inner products sum over blocks in ascending order, or descending after oracle_set_sum_order(1): the same
+ * algorithm with other rounding, which measures how far rounding alone separates two runs after m half-steps. */
+static int g_reverse_sums = 0;
+void oracle_set_sum_order(int reverse) { g_reverse_sums = reverse != 0; }
+#define SUM_BLK(k) (g_reverse_sums ? p->nnzb - 1 - (k) : (k))
 typedef struct { double re, im; } cplx;
 static cplx cdiv(cplx a, cplx b) {
     double const den = b.re * b.re + b.im * b.im;
@@ -449,7 +454,8 @@ int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit,
 #define SI(blk, i, j) oracle_shadow(seed, (((blk) * 2 + 1) * 64 + (i)) * 64 + (j))
     /* rho = dot(rs, b), tau = ||b|| */
     memset(acc, 0, 3 * (size_t)ncol * 8);
-    for (uint64_t blk = 0; blk < p->nnzb; ++blk)
+    for (uint64_t kb = 0; kb < p->nnzb; ++kb) {
+        uint64_t const blk = SUM_BLK(kb);
         for (int i = 0; i < 64; ++i)
             for (int j = 0; j < 64; ++j) {
                 uint32_t const q = COL(blk, j);
@@ -458,6 +464,7 @@ int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit,
                 acc[ncol + q] += sr * br + si * bi;
                 acc[2 * ncol + q] += sr * bi - si * br;
             }
+    }
     for (uint32_t q = 0; q < ncol; ++q) {
         tau[q] = sqrt(acc[q]);
         rho[q].re = acc[ncol + q]; rho[q].im = acc[2 * ncol + q];
@@ -477,7 +484,8 @@ int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit,
         if (!active) break;
         if ((m & 1) == 0) {
             memset(acc, 0, 2 * (size_t)ncol * 8);
-            for (uint64_t blk = 0; blk < p->nnzb; ++blk)
+            for (uint64_t kb = 0; kb < p->nnzb; ++kb) {
+                uint64_t const blk = SUM_BLK(kb);
                 for (int i = 0; i < 64; ++i)
                     for (int j = 0; j < 64; ++j) {
                         uint32_t const q = COL(blk, j);
@@ -485,6 +493,7 @@ int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit,
                         acc[q] += sr * vr + si * vi;
                         acc[ncol + q] += sr * vi - si * vr;
                     }
+            }
             for (uint32_t q = 0; q < ncol; ++q) {
                 if (status[q]) { alpha[q].re = alpha[q].im = 0; continue; }
                 cplx sg = {acc[q], acc[ncol + q]};
@@ -492,7 +501,8 @@ int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit,
                 alpha[q] = cdiv(rho[q], sg);
             }
             memset(acc, 0, (size_t)ncol * 8);
-            for (uint64_t blk = 0; blk < p->nnzb; ++blk)
+            for (uint64_t kb = 0; kb < p->nnzb; ++kb) {
+                uint64_t const blk = SUM_BLK(kb);
                 for (int i = 0; i < 64; ++i)
                     for (int j = 0; j < 64; ++j) {
                         cplx const a = alpha[COL(blk, j)];
@@ -505,11 +515,13 @@ int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit,
                         RE(w, blk, i, j) = wr; IM(w, blk, i, j) = wi;
                         acc[COL(blk, j)] += wr * wr + wi * wi;
                     }
+            }
         } else {
             memset(Aodd, 0, n * 8);
             oracle_action(s, -1, Aodd, u);
             memset(acc, 0, 3 * (size_t)ncol * 8);
-            for (uint64_t blk = 0; blk < p->nnzb; ++blk)
+            for (uint64_t kb = 0; kb < p->nnzb; ++kb) {
+                uint64_t const blk = SUM_BLK(kb);
                 for (int i = 0; i < 64; ++i)
                     for (int j = 0; j < 64; ++j) {
                         uint32_t const q = COL(blk, j);
@@ -523,6 +535,7 @@ int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit,
                         acc[ncol + q] += sr * wr + si * wi;
                         acc[2 * ncol + q] += sr * wi - si * wr;
                     }
+            }
         }
         /* theta, c, tau, eta, d coefficient, convergence; rho and beta on odd steps */
         for (uint32_t q = 0; q < ncol; ++q) {

@@ -75,6 +75,7 @@ double oracle_shadow(uint64_t seed, uint64_t idx);
 /* TFQMR (same algorithm, scalars and shadow as the device solver) for B = unit columns on the
  * diagonal blocks; X [nnzb][2][64][64]; returns the number of half-steps; resid[64 nCols] receives the
  * true relative residual per column. */
+void oracle_set_sum_order(int reverse);
 int oracle_tfqmr(oracle_state* s, const uint32_t* subset, double tol, int maxit, uint64_t seed, double* X,
                  double* resid);
 

@@ -49,6 +49,8 @@ def lib():
         L.oracle_action.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.oracle_shadow.restype = C.c_double
         L.oracle_shadow.argtypes = [C.c_uint64, C.c_uint64]
+        L.oracle_set_sum_order.argtypes = [C.c_int]
+        L.oracle_set_sum_order.restype = None
         L.oracle_tfqmr.restype = C.c_int
         L.oracle_tfqmr.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p]
         _lib = L
@@ -126,9 +128,15 @@ class Oracle:
         lib().oracle_action(self.h, which, _p(y), _p(x))
         return y
 
-    def tfqmr(self, tol=1e-9, maxit=400, seed=0x5EEDA43C):
+    def tfqmr(self, tol=1e-9, maxit=400, seed=0x5EEDA43C, reverse_sums=False):
+        """TFQMR from X = 0 for the unit columns; reverse_sums sums the inner products over blocks in descending
+        order (the same algorithm with other rounding)."""
         X = np.zeros((self.nnzb, 2, 64, 64))
         res = np.zeros(64 * self.nCols)
         sub = self._keep["SUBSET"]
-        m = lib().oracle_tfqmr(self.h, _p(sub), tol, maxit, seed, _p(X), _p(res))
+        lib().oracle_set_sum_order(1 if reverse_sums else 0)
+        try:
+            m = lib().oracle_tfqmr(self.h, _p(sub), tol, maxit, seed, _p(X), _p(res))
+        finally:
+            lib().oracle_set_sum_order(0)
         return X, res, m

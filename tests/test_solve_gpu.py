@@ -67,7 +67,38 @@ def test_fp64_solve_follows_oracle_tfqmr(device, name, mk, kw, E, k):
     assert np.abs(rho - rho_x).max() <= 1e-15 * np.abs(rho_x).max()
 
 
-@pytest.mark.parametrize("name,mk,kw,E,k", CASES)
+# the plain FP64 workload on the reference's CPU path: the 8^3-block periodic cell of 4 atoms (numax 3), Gamma,
+# E = -0.5 + 0.05i Ha, one source block of 64 columns, truncation radius 6 Bohr (untruncated: 512 blocks)
+SMALL_PERIODIC = ("periodic8", lambda: S.small_periodic(), dict(r_trunc=6.0), complex(-0.5, 0.05), (0.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("halfsteps", [16, 64])
+def test_fixed_halfsteps_follow_oracle(device, halfsteps):
+    """With tolerance 1e-30 neither side stops early: the device FP64 tfQMR and the oracle's CPU TFQMR run the same
+    number of half-steps with the same seeded shadow residual.  Over 16 half-steps their iterates agree to 1e-12
+    relative.  Later, rounding alone separates any two runs of TFQMR (a short recurrence that does not correct
+    itself): the oracle against itself with the inner products summed in reverse block order is 1.3e-10 apart
+    after 64 half-steps.  The device must stay within 10 times that envelope, measured here on the same inputs."""
+    name, mk, kw, E, k = SMALL_PERIODIC
+    plan = G.GreenPlan(mk(), **kw)
+    plan.set_inputs(E, k)
+    print(f"fixed {halfsteps} half-steps on {name}: nnzb {plan.nnzb}", flush=True)
+    X, rho, st = run(plan, device, fp_bits=64, tolerance=1e-30, max_iterations=halfsteps, max_refinements=0)
+    ora = O.Oracle(plan, E, k)
+    Xo, _, m_o = ora.tfqmr(tol=1e-30, maxit=halfsteps, seed=SEED)
+    diff = np.abs(X - Xo).max() / np.abs(Xo).max()
+    assert st["iterations"] == m_o == halfsteps
+    if halfsteps <= 16:
+        print(f"  max |X - X_oracle| / max |X| = {diff:.2e}")
+        assert diff <= 1e-12
+    else:
+        Xr, _, _ = ora.tfqmr(tol=1e-30, maxit=halfsteps, seed=SEED, reverse_sums=True)
+        env = np.abs(Xr - Xo).max() / np.abs(Xo).max()
+        print(f"  max |X - X_oracle| / max |X| = {diff:.2e}; oracle vs reversed sums {env:.2e}")
+        assert diff <= 10 * env
+
+
+@pytest.mark.parametrize("name,mk,kw,E,k", CASES + [SMALL_PERIODIC])
 def test_solve_true_residual_meets_tolerance(device, name, mk, kw, E, k):
     plan = G.GreenPlan(mk(), **kw)
     plan.set_inputs(E, k)
@@ -155,3 +186,30 @@ def test_cocg_with_every_tile_shape(device, kind, columns, lines):
             assert sc["converged_columns"] == sc["total_columns"], (name, fp)
             assert r.max() <= 1e-9, (name, fp, r.max())
             assert err < 1e-6, (name, fp, err)
+
+
+def test_solver_state_is_released_between_plans(device):
+    """Two plans solved back to back: each solve's state stays with its plan until released; release returns it to
+    the device (what lets one plan per step run without accumulating work vectors), and the state never exceeds
+    the library's own estimate of the solve's memory."""
+    import torch
+    sysm = S.make_system(11, (12, 12, 12), ngauss=8, natoms=6, numax=3, bc=(1, 1, 1))
+    held = []
+    for r, E in ((6.0, complex(-0.4, 0.1)), (5.0, complex(-0.3, 0.05))):
+        plan = G.GreenPlan(sysm, r_trunc=r, sources=[(6, 6, 6)] * 2, source_radii=[r, r - 1.0])
+        plan.set_inputs(E)
+        for fp, method in ((32, "cocg"), (64, "tfqmr")):
+            est, _ = plan.solve_memory(fp, method)
+            _, st = plan.solve(fp_bits=fp, tolerance=1e-9, max_iterations=3000, max_refinements=6, method=method)
+            assert st["converged_columns"] == st["total_columns"]
+            used = plan.sizes.solver_bytes
+            print(f"plan nnzb {plan.nnzb} fp{fp} {method}: solver {used / 1e6:.1f} MB, estimate {est / 1e6:.1f} MB")
+            assert 0 < used <= est
+            torch.cuda.synchronize()
+            free0 = torch.cuda.mem_get_info()[0]
+            plan.release_solver()
+            free1 = torch.cuda.mem_get_info()[0]
+            assert plan.sizes.solver_bytes == 0
+            held.append(used)
+            assert free1 - free0 >= 0.9 * used, (free0, free1, used)
+        plan.close()
