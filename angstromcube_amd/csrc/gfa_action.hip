@@ -30,6 +30,9 @@ namespace gfa {
 namespace {
 
 constexpr int NBLK = 2 * 64 * 64;  // elements of one block
+// the autotuner also times the two engines of the SHO projection (the matrix-core one is off until its parity
+// is established on the device)
+constexpr bool TUNE_PROJECTION_ENGINE = false;
 #ifndef LOCAL_TILE
 #define LOCAL_TILE 6   // xy tile edge (blocks) of the local kernel's processing order (5 to 7 measured best)
 #endif
@@ -102,6 +105,14 @@ __device__ __forceinline__ void stage_hermite(real_t (*H)[3][4], float (*xi2)[4]
     double const* src = htab + size_t(slot) * 96;
     for (int e = tid; e < (L + 1) * 12; e += T) H[e / 12][(e % 12) / 4][e % 4] = real_t(src[e]);
     if (tid < 12) xi2[tid / 4][tid % 4] = xtab[size_t(slot) * 12 + tid];
+}
+
+// is point (y, x) of z plane zq inside the projection sphere (the 3D mask evaluated in float from xi^2)
+__device__ __forceinline__ bool sho_inside(float const (*xi2)[4], float R2, int zq, int y, int xx) {
+    float const d2z = xi2[2][zq] - R2;
+    float const d2yz = xi2[1][y] + d2z;
+    float const d2 = xi2[0][xx] + d2yz;
+    return (d2z < 0) && (d2yz < 0) && (d2 < 0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -185,6 +196,107 @@ __global__ __launch_bounds__(128) void k_sho_project(
     real_t* out = partial + size_t(chunk_coef[ich]) * 128 + p * 64 + j;
 #pragma unroll
     for (int s = 0; s < n3; ++s) out[size_t(s) * 128] = czyx[s];
+}
+
+// ---------------------------------------------------------------------------------------------
+// 1b. projection on the matrix cores (the dense form): per block of the chunk, C[s][n] += sum_i P[s][i] X[i][n]
+//     with P[s][i] = H_x[ix_s](x_i) H_y[iy_s](y_i) H_z[iz_s](z_i) inside the projection sphere (0 outside), i the 64
+//     points, n = plane * 64 + column.  P is built in LDS per block, X comes straight from global memory as the
+//     B operand.  4 waves; wave w owns the columns n = 32 w .. 32 w + 31 (two 16-column tiles) and every
+//     16-coefficient tile.  Operand layout of the 16x16x4 MFMA (64 lanes): A[m][k] from lane m + 16 k, B[k][n]
+//     from lane n + 16 k; accumulator r of a lane holds D[row][lane % 16] with row = 4 (lane / 16) + r in FP32 and
+//     row = lane / 16 + 4 r in FP64 (measured on gfx950).
+// ---------------------------------------------------------------------------------------------
+template <typename real_t> struct Mfma;
+template <> struct Mfma<float> {
+    typedef float __attribute__((ext_vector_type(4))) acc_t;
+    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kl, int r) { return 4 * kl + r; }
+};
+template <> struct Mfma<double> {
+    typedef double __attribute__((ext_vector_type(4))) acc_t;
+    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kl, int r) { return kl + 4 * r; }
+};
+
+template <typename real_t, int L>
+__global__ __launch_bounds__(256) void k_sho_project_mfma(
+    real_t* __restrict__ partial, real_t const* __restrict__ x, ProjChunk const* __restrict__ chunks,
+    uint32_t const* __restrict__ bucket, uint32_t const* __restrict__ chunk_coef,
+    uint32_t const* __restrict__ prj_blocks, double rproj, uint8_t const* __restrict__ col_active,
+    double const* __restrict__ htab, float const* __restrict__ xtab, uint32_t const* __restrict__ prj_slot) {
+    using acc_t = typename Mfma<real_t>::acc_t;
+    constexpr int NC = Sho<L>::n3, MT = (NC + 15) / 16, NCP = 16 * MT;
+    __shared__ real_t P[64 * NCP];           // [point][coefficient], zero beyond NC and outside the sphere
+    __shared__ real_t H[L + 1][3][4];
+    __shared__ float xi2[3][4];
+    __shared__ uint16_t sxyz[NCP];           // coefficient s -> ix | iy << 4 | iz << 8
+    uint32_t const ich = bucket[blockIdx.x];
+    ProjChunk const ch = chunks[ich];
+    if (col_active && !col_active[ch.col]) return;  // a retired column group
+    int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) {
+        int s = 0;
+        for (int iz = 0; iz <= L; ++iz)
+            for (int iy = 0; iy <= L - iz; ++iy)
+                for (int ix = 0; ix <= L - iz - iy; ++ix) sxyz[s++] = uint16_t(ix | (iy << 4) | (iz << 8));
+        for (; s < NCP; ++s) sxyz[s] = 0xFFFF;
+    }
+    float const R2 = float(rproj * rproj);
+    acc_t acc[MT][2];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[mt][t] = acc_t{0, 0, 0, 0};
+    int const p = w >> 1, jb = (w & 1) * 32 + (lane & 15), kl = lane >> 4;   // B operand: column jb (+16), row kl
+    for (uint32_t b = ch.begin; b < ch.end; ++b) {
+        uint32_t const inzb = prj_blocks[b];
+        // the B operands of the whole block (16 steps of 4 points, 2 column tiles), issued before P is built
+        real_t const* xb = x + size_t(inzb) * NBLK + p * 4096 + jb;
+        real_t bv[16][2];
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) {
+            bv[ks][0] = xb[(ks * 4 + kl) * 64];
+            bv[ks][1] = xb[(ks * 4 + kl) * 64 + 16];
+        }
+        __syncthreads();   // P of the previous block is no longer read
+        stage_hermite<real_t>(H, xi2, htab, xtab, prj_slot[b], L, tid, 256);
+        __syncthreads();
+        for (int e = tid; e < 64 * NCP; e += 256) {
+            int const i = e / NCP, s = e % NCP;
+            int const xx = i & 3, yy = (i >> 2) & 3, zz = i >> 4;
+            uint16_t const c = sxyz[s];
+            real_t v = 0;
+            if (c != 0xFFFF && sho_inside(xi2, R2, zz, yy, xx))
+                v = H[c & 15][0][xx] * H[(c >> 4) & 15][1][yy] * H[c >> 8][2][zz];
+            P[e] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                real_t const av = P[(ks * 4 + kl) * NCP + mt * 16 + (lane & 15)];
+                acc[mt][0] = Mfma<real_t>::mac(av, bv[ks][0], acc[mt][0]);
+                acc[mt][1] = Mfma<real_t>::mac(av, bv[ks][1], acc[mt][1]);
+            }
+        }
+    }
+    real_t* out = partial + size_t(chunk_coef[ich]) * 128 + p * 64 + jb;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int const s = mt * 16 + Mfma<real_t>::row(kl, r);
+            if (s < NC) {
+                out[size_t(s) * 128] = acc[mt][0][r];
+                out[size_t(s) * 128 + 16] = acc[mt][1][r];
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -387,13 +499,6 @@ __device__ __forceinline__ void sho_plane_any(real_t const (*H)[3][4], real_t co
     }
 }
 
-// is point (y, x) of z plane zq inside the projection sphere (the 3D mask evaluated in float from xi^2)
-__device__ __forceinline__ bool sho_inside(float const (*xi2)[4], float R2, int zq, int y, int xx) {
-    float const d2z = xi2[2][zq] - R2;
-    float const d2yz = xi2[1][y] + d2z;
-    float const d2 = xi2[0][xx] + d2yz;
-    return (d2z < 0) && (d2yz < 0) && (d2 < 0);
-}
 
 // SHO add-back of one atom image (numax L <= LMAX) into this thread's z plane zq of an accumulator in memory
 // (LDS or global) whose points are ostride apart: ys[(zq*16 + y*4 + x) * ostride]
@@ -879,6 +984,346 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     plane_finish<real_t, JW>(a, blk, slice, p, zq, jl, acc, own, xo, xs, H, xi2);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Line variant of the local kernel (kind 2).  One work-group per (block, JW-column slice).  A lane owns
+// CPL = 16 / sizeof(real_t) adjacent columns of one plane, so that every global load and store moves 16 bytes per
+// lane.  Thread (cq, p, l) owns line l = 0..15 of plane p in each direction: the x-line (y, z) = (l % 4, l / 4),
+// the y-line (x, z) and the z-line (x, y) of the same two indices.  Every neighbour value the three stencils need
+// is loaded once per work-group, straight into registers, and all of them are issued before the first stencil;
+// the block's own y- and z-lines come from the LDS copy of the slice.  The y and z stencil sums meet the x-line
+// accumulator (potential, x stencil, SHO add-back) in a second LDS buffer.
+// ---------------------------------------------------------------------------------------------
+template <typename real_t>
+struct alignas(16) Lane {   // the 16 bytes of adjacent columns a lane owns at one point
+    static constexpr int n = 16 / sizeof(real_t);
+    real_t v[n];
+};
+
+__device__ __forceinline__ Lane<float> ld16(float const* p) {
+    float4 const t = *reinterpret_cast<float4 const*>(p);
+    return Lane<float>{{t.x, t.y, t.z, t.w}};
+}
+__device__ __forceinline__ Lane<double> ld16(double const* p) {
+    double2 const t = *reinterpret_cast<double2 const*>(p);
+    return Lane<double>{{t.x, t.y}};
+}
+__device__ __forceinline__ void st16(float* p, Lane<float> const& v) {
+    *reinterpret_cast<float4*>(p) = float4{v.v[0], v.v[1], v.v[2], v.v[3]};
+}
+__device__ __forceinline__ void st16(double* p, Lane<double> const& v) {
+    *reinterpret_cast<double2*>(p) = double2{v.v[0], v.v[1]};
+}
+// 16 bytes at element e of the uniform base b plus the lane's byte offset off (the load's own addressing mode)
+template <typename real_t>
+__device__ __forceinline__ Lane<real_t> ldv(real_t const* __restrict__ b, int e, uint32_t off) {
+    return ld16(reinterpret_cast<real_t const*>(reinterpret_cast<char const*>(b + e) + off));
+}
+
+// acc[q] += the off-centre stencil of direction d at point q of a line, for every column of the lane: the window
+// is the own line (centre) and NH neighbour lines on each side (w[side][k][q], k = 0 nearest)
+template <typename real_t, int NH>
+__device__ __forceinline__ void line_stencil(Lane<real_t> (&acc)[4], LocalArgs<real_t> const& a, int d,
+                                             Lane<real_t> const (&own)[4], Lane<real_t> const (&w)[2][NH][4]) {
+    constexpr int NW = 4 * (2 * NH + 1);
+#pragma unroll
+    for (int c = 0; c < Lane<real_t>::n; ++c) {
+        real_t win[NW];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            win[4 * NH + q] = own[q].v[c];
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                win[4 * (NH - 1 - k) + q] = w[0][k][q].v[c];
+                win[4 * (NH + 1 + k) + q] = w[1][k][q].v[c];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) stencil<real_t, NH, NW>(acc[q].v[c], a, d, win, 4 * NH + q);
+    }
+}
+
+// one z layer of sho_xline: the M = L - iz triangle of coefficients (iy + ix <= M, ix fastest) times
+// H_z[iz](z) H_y[iy](y), accumulated into t[column][ix]
+template <typename real_t, int L, int M>
+__device__ __forceinline__ void xline_layer(real_t (&t)[Lane<real_t>::n][L + 1], real_t const* cs, int cstride,
+                                            real_t hz, real_t const (*H)[3][4], int y) {
+    int s = 0;
+#pragma unroll
+    for (int iy = 0; iy <= M; ++iy) {
+        real_t const hzy = hz * H[iy][1][y];
+#pragma unroll
+        for (int ix = 0; ix <= M - iy; ++ix, ++s) {
+            Lane<real_t> const cv = ld16(cs + s * cstride);
+#pragma unroll
+            for (int c = 0; c < Lane<real_t>::n; ++c) t[c][ix] += cv.v[c] * hzy;
+        }
+    }
+}
+
+// SHO add-back of one atom image (numax L) into the x-line (y, z) of a lane: the coefficients c[iz][iy][ix]
+// (ix fastest) are contracted with H_z(z) H_y(y) first, t[ix], then t with H_x at the 4 points of the line.
+// The z layers run in a rolled loop, so that one layer of coefficients at a time is loaded ahead.
+// cs points to coefficient 0 of the lane's plane and first column; consecutive coefficients are cstride apart.
+template <typename real_t, int L>
+__device__ __forceinline__ void sho_xline(Lane<real_t> (&acc)[4], real_t const (*H)[3][4], float const (*xi2)[4],
+                                          float R2, real_t const* cs, int cstride, int y, int z) {
+    constexpr int CPL = Lane<real_t>::n;
+    real_t t[CPL][L + 1];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+#pragma unroll
+        for (int ix = 0; ix <= L; ++ix) t[c][ix] = 0;
+#pragma unroll 1
+    for (int iz = 0; iz <= L; ++iz) {
+        real_t const hz = H[iz][2][z];
+        switch (L - iz) {
+            case 0: xline_layer<real_t, L, 0>(t, cs, cstride, hz, H, y); break;
+            case 1: if constexpr (L >= 1) xline_layer<real_t, L, 1>(t, cs, cstride, hz, H, y); break;
+            case 2: if constexpr (L >= 2) xline_layer<real_t, L, 2>(t, cs, cstride, hz, H, y); break;
+            case 3: if constexpr (L >= 3) xline_layer<real_t, L, 3>(t, cs, cstride, hz, H, y); break;
+            case 4: if constexpr (L >= 4) xline_layer<real_t, L, 4>(t, cs, cstride, hz, H, y); break;
+            default: if constexpr (L >= 5) xline_layer<real_t, L, 5>(t, cs, cstride, hz, H, y); break;
+        }
+        cs += ((L - iz + 1) * (L - iz + 2)) / 2 * cstride;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        bool const in = sho_inside(xi2, R2, z, y, q);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            real_t v = 0;
+#pragma unroll
+            for (int ix = 0; ix <= L; ++ix) v += t[c][ix] * H[ix][0][q];
+            acc[q].v[c] += in ? v : real_t(0);
+        }
+    }
+}
+
+// the neighbour lines of direction d (NH blocks per side) of the lane's line starting at point i0, plane p and
+// first column j0, into w; a periodic-image neighbour (uniform over the work-group) gets its Bloch phase,
+// f x_p - (+/-) g x_other, which waits for its loads (image blocks are few)
+template <typename real_t, int NH, bool IMG>
+__device__ __forceinline__ void load_lines(LocalArgs<real_t> const& a, int d, uint64_t blk, int p, int i0, int j0,
+                                           Lane<real_t> (&w)[2][NH][4]) {
+    int const stride = (d == 0) ? 1 : ((d == 1) ? 4 : 16);
+    uint32_t const off = uint32_t(p * 4096 + i0 * 64 + j0) * sizeof(real_t);
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+            uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
+            real_t const* nb = (code == -1) ? a.zero : a.x + idx * NBLK;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[sd][k][q] = ldv(nb, q * stride * 64, off);
+        }
+    if (IMG) {
+        uint32_t const offo = uint32_t((1 - p) * 4096 + i0 * 64 + j0) * sizeof(real_t);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+                if (code <= -2) {
+                    real_t const* nb = a.x + uint64_t(-code - 2) * NBLK;
+                    real_t const fr = real_t(a.phase[d][sd][0]), fi = real_t(a.phase[d][sd][1]) * real_t(1 - 2 * p);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        Lane<real_t> const o = ldv(nb, q * stride * 64, offo);
+#pragma unroll
+                        for (int c = 0; c < Lane<real_t>::n; ++c) w[sd][k][q].v[c] = fr * w[sd][k][q].v[c] - fi * o.v[c];
+                    }
+                }
+            }
+    }
+}
+
+// PF: directions whose neighbour lines are issued before the first barrier (3: all; 2: x and y, z after the x
+// stencil; 1: x, then y after the x stencil and z after the y stencil) -- loads in flight against occupancy
+template <typename real_t, int NH, bool IMG, int JW, int PF>
+__global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs<real_t> a) {
+    using L16 = Lane<real_t>;
+    constexpr int CPL = L16::n, LQ = JW / CPL, T = LQ * 32;
+    constexpr int NSLICE = 64 / JW;
+    __shared__ __align__(16) real_t xs[2 * 64 * JW];   // the slice [plane][point][column]
+    __shared__ __align__(16) real_t ya[2 * 64 * JW];   // y + z stencil sums; then the coefficients of an image
+    __shared__ real_t H[8][3][4];
+    __shared__ float xi2[3][4];
+
+    uint64_t lin = blockIdx.x;
+    if (a.per_xcd) lin = uint64_t(blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
+    uint64_t const item = lin % a.nitems;   // slice-major, as in k_plane
+    int const slice = int(lin / a.nitems);
+    if (slice >= NSLICE) return;  // uniform over the work-group
+    uint64_t const blk = a.order[item];
+    int const tid = threadIdx.x, cq = tid % LQ, p = (tid / LQ) & 1, l = tid / (2 * LQ);
+    int const la = l & 3, lb = l >> 2;
+    int const jc = cq * CPL, j0 = slice * JW + jc;
+    // first points of the lines: x-line (y, z) = (la, lb), y-line (x, z) = (la, lb), z-line (x, y) = (la, lb)
+    int const ix0 = lb * 16 + la * 4, iy0 = lb * 16 + la, iz0 = lb * 4 + la;
+
+    L16 wx[2][NH][4], wy[2][NH][4], wz[2][NH][4];   // neighbour lines per direction
+    load_lines<real_t, NH, IMG>(a, 0, blk, p, ix0, j0, wx);
+    if constexpr (PF >= 2) load_lines<real_t, NH, IMG>(a, 1, blk, p, iy0, j0, wy);
+    if constexpr (PF >= 3) load_lines<real_t, NH, IMG>(a, 2, blk, p, iz0, j0, wz);
+    // own x-line, to registers and to the LDS copy of the slice
+    L16 own[4];
+    {
+        real_t const* src = a.x + blk * NBLK + p * 4096 + j0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) own[q] = ld16(src + (ix0 + q) * 64);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) st16(xs + (p * 64 + ix0 + q) * JW + jc, own[q]);
+    }
+    __syncthreads();
+
+    // ---- potential and x stencil on the x-line ----
+    L16 acc[4];
+    {
+        L16 oth[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) oth[q] = ld16(xs + ((1 - p) * 64 + ix0 + q) * JW + jc);
+        int32_t const iv = a.veff[blk];
+        real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
+        double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + ix0;
+        uint32_t const colE = a.colindx[blk];
+        real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
+        real_t const Vim = E_im * real_t(1 - 2 * p);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            real_t const dg = real_t(Vb[q]) * vm + (a.cdiag - E_re);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) acc[q].v[c] = dg * own[q].v[c] + Vim * oth[q].v[c];
+        }
+        line_stencil<real_t, NH>(acc, a, 0, own, wx);
+    }
+    if constexpr (PF < 2) load_lines<real_t, NH, IMG>(a, 1, blk, p, iy0, j0, wy);
+    if constexpr (PF == 2) load_lines<real_t, NH, IMG>(a, 2, blk, p, iz0, j0, wz);
+
+    // ---- y stencil on the y-line, into ya ----
+    {
+        L16 o[4], s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            o[q] = ld16(xs + (p * 64 + iy0 + 4 * q) * JW + jc);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) s[q].v[c] = 0;
+        }
+        line_stencil<real_t, NH>(s, a, 1, o, wy);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) st16(ya + (p * 64 + iy0 + 4 * q) * JW + jc, s[q]);
+    }
+    if constexpr (PF < 2) load_lines<real_t, NH, IMG>(a, 2, blk, p, iz0, j0, wz);
+    __syncthreads();
+    // ---- z stencil on the z-line, added into ya ----
+    {
+        L16 o[4], s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            o[q] = ld16(xs + (p * 64 + iz0 + 16 * q) * JW + jc);
+            s[q] = ld16(ya + (p * 64 + iz0 + 16 * q) * JW + jc);
+        }
+        line_stencil<real_t, NH>(s, a, 2, o, wz);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) st16(ya + (p * 64 + iz0 + 16 * q) * JW + jc, s[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        L16 const s = ld16(ya + (p * 64 + ix0 + q) * JW + jc);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[q].v[c] += s.v[c];
+    }
+
+    // ---- SHO add-back of the atom images (numax <= 5) touching the block; coefficients staged in ya ----
+    uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
+    if (b1 > b0) {
+        uint32_t const col = a.colindx[blk];
+        float const R2 = float(a.hgrid[3] * a.hgrid[3]);
+        constexpr int VPR = JW / CPL;
+        for (uint32_t b = b0; b < b1; ++b) {
+            uint32_t const img = a.add_index[b];
+            int const L = a.image_lmax[img];
+            if (L > 5) continue;  // done by k_sho_add_wide
+            int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
+            __syncthreads();   // ya (the stencil sums, or the previous image) is no longer read
+            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
+            {   // [coefficient][plane][JW]
+                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
+                for (int v = tid; v < 2 * nc * VPR; v += T) {
+                    int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
+                    st16(ya + row * JW + cv * CPL, ld16(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * CPL));
+                }
+            }
+            __syncthreads();
+            real_t const* cs = ya + p * JW + jc;
+            switch (L) {
+                case 0: sho_xline<real_t, 0>(acc, H, xi2, R2, cs, 2 * JW, la, lb); break;
+                case 1: sho_xline<real_t, 1>(acc, H, xi2, R2, cs, 2 * JW, la, lb); break;
+                case 2: sho_xline<real_t, 2>(acc, H, xi2, R2, cs, 2 * JW, la, lb); break;
+                case 3: sho_xline<real_t, 3>(acc, H, xi2, R2, cs, 2 * JW, la, lb); break;
+                case 4: sho_xline<real_t, 4>(acc, H, xi2, R2, cs, 2 * JW, la, lb); break;
+                default: sho_xline<real_t, 5>(acc, H, xi2, R2, cs, 2 * JW, la, lb); break;
+            }
+        }
+    }
+
+    // ---- store the x-line ----
+    {
+        real_t* dst = a.y + blk * NBLK + p * 4096 + j0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) st16(dst + (ix0 + q) * 64, acc[q]);
+    }
+    if (a.dot) {
+        // (x, y) = sum_i x_i y_i per column: plane 0 holds x_r, y_r (+ x_i), plane 1 x_i, y_i (+ x_r):
+        // Re = sum x_r y_r - sum x_i y_i, Im = sum x_i y_r + sum x_r y_i; 4 products in the vector precision,
+        // then FP64 over the 32 lines and planes of the work-group in a fixed order
+        real_t re_t[CPL], im_t[CPL];   // x of both planes at the x-line from the LDS copy of the slice
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) re_t[c] = im_t[c] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            L16 const xp = ld16(xs + (p * 64 + ix0 + q) * JW + jc), xo = ld16(xs + ((1 - p) * 64 + ix0 + q) * JW + jc);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                re_t[c] += xp.v[c] * acc[q].v[c];
+                im_t[c] += xo.v[c] * acc[q].v[c];
+            }
+        }
+        // the lanes of one column group within a wave (lane bits LQ and up: plane and line) are summed with
+        // shuffles, then the waves' partial sums in LDS ([wave][re, im][JW], fits xs) in wave order
+        constexpr int NWAVE = T / 64;
+        double re[CPL], im[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            re[c] = (p == 1) ? -double(re_t[c]) : double(re_t[c]);
+            im[c] = double(im_t[c]);
+#pragma unroll
+            for (int o = LQ; o < 64; o <<= 1) {
+                re[c] += __shfl_xor(re[c], o, 64);
+                im[c] += __shfl_xor(im[c], o, 64);
+            }
+        }
+        __syncthreads();   // xs is free
+        double* red = reinterpret_cast<double*>(xs);
+        int const wave = tid >> 6, lane = tid & 63;
+        if (lane < LQ) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                red[(wave * 2 + 0) * JW + jc + c] = re[c];
+                red[(wave * 2 + 1) * JW + jc + c] = im[c];
+            }
+        }
+        __syncthreads();
+        if (tid < JW) {
+            double sr = 0, si = 0;
+            for (int w = 0; w < NWAVE; ++w) { sr += red[(w * 2 + 0) * JW + tid]; si += red[(w * 2 + 1) * JW + tid]; }
+            a.dot[(blk * 2 + 0) * 64 + slice * JW + tid] = sr;
+            a.dot[(blk * 2 + 1) * 64 + slice * JW + tid] = si;
+        }
+    }
+}
+
 // SHO add-back of the atom images with numax 6 or 7, y += P d, on the blocks listed in blocks[]: one work-group of
 // 512 threads = (column j, plane p, z plane zq) per block, taking the block's wide images one after the other, so
 // that no two work-groups update the same block.  Runs after the local kernel (which adds numax <= 5).
@@ -1252,11 +1697,28 @@ __global__ __launch_bounds__(64) void k_grad_atoms(double* __restrict__ gmat, do
 
 // Local-kernel tile shapes.  kind 0: k_local (LDS accumulator), JW columns per work-group (16 or 32), LB
 // kinetic lines per thread in flight (1, 2, 4).  kind 1: k_plane (register accumulator), JW = 32 or 64.
+// kind 2: k_lines (16-byte lanes, line ownership), JW = 16 or 32, or 64 in FP32; LB = directions of neighbour
+// lines issued up front (3 or 2; 16th-order stencils issue one direction at a time).
 template <typename real_t, int NH, bool IMG>
 static void launch_local(int kind, int jw, int lb, dim3 g, hipStream_t stream, LocalArgs<real_t> const& a) {
 #define KL(JW_, LB_) hipLaunchKernelGGL((k_local<real_t, NH, IMG, JW_, LB_>), g, dim3(8 * JW_), 0, stream, a)
 #define KP(JW_) hipLaunchKernelGGL((k_plane<real_t, NH, IMG, JW_>), g, dim3(8 * JW_), 0, stream, a)
-    if (kind == 1) {
+#define KN(JW_, PF_) hipLaunchKernelGGL((k_lines<real_t, NH, IMG, JW_, PF_>), g, dim3((JW_ / Lane<real_t>::n) * 32), 0, stream, a)
+    if (kind == 2) {   // lb: directions prefetched (3 or 2; 16th-order stencils always 1)
+        if constexpr (NH == 2) {
+            if (jw == 16) KN(16, 1);
+            else if (jw == 64) { if constexpr (sizeof(real_t) == 4) KN(64, 1); }
+            else KN(32, 1);
+        } else if (lb == 2) {
+            if (jw == 16) KN(16, 2);
+            else if (jw == 64) { if constexpr (sizeof(real_t) == 4) KN(64, 2); }
+            else KN(32, 2);
+        } else {
+            if (jw == 16) KN(16, 3);
+            else if (jw == 64) { if constexpr (sizeof(real_t) == 4) KN(64, 3); }
+            else KN(32, 3);
+        }
+    } else if (kind == 1) {
         if (jw == 64) KP(64); else KP(32);
     } else if (jw == 16) {
         if (lb == 1) KL(16, 1); else if (lb == 4) KL(16, 4); else KL(16, 2);
@@ -1265,6 +1727,7 @@ static void launch_local(int kind, int jw, int lb, dim3 g, hipStream_t stream, L
     }
 #undef KL
 #undef KP
+#undef KN
 }
 
 template <typename real_t>
@@ -1274,20 +1737,23 @@ template <> Workspace<float>& workspace<float>(DevicePlan& d) { return d.ws32; }
 
 template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
-                  std::string& err, int parts, double* dot) {
-    auto& ws = workspace<real_t>(d);
-    // every buffer of this precision is allocated on the first call, so that later calls (also under graph
-    // capture) allocate nothing; parts = 0 only allocates
-    if (d.nAtomImages > 0 && !ws.dcoef) {
-        if (int r = dalloc(d, ws.partial, size_t(d.total_chunk_coefs) * 128, err)) return r;
-        if (int r = dalloc(d, ws.dcoef, size_t(d.natomcoeffs) * d.nCols * 128, err)) return r;
-    }
-    if (d.nAtomImages > 0 && d.nchunks > 0 && d.max_atom_nc > 64 && !d.cscratch) {
-        if (int r = dalloc(d, d.cscratch, size_t(d.ncoeffs) * d.nCols * 128, err)) return r;
-    }
-    if (!ws.zero) {
-        if (int r = dalloc(d, ws.zero, NBLK, err)) return r;
-        HIP_TRY(hipMemsetAsync(ws.zero, 0, NBLK * sizeof(real_t), stream));
+                  std::string& err, int parts, double* dot, Workspace<real_t>* ext, double* ext_cscratch) {
+    auto& ws = ext ? *ext : workspace<real_t>(d);
+    double* const cscratch = ext ? ext_cscratch : d.cscratch;
+    if (!ext) {
+        // every buffer of this precision is allocated on the first call, so that later calls (also under graph
+        // capture) allocate nothing; parts = 0 only allocates
+        if (d.nAtomImages > 0 && !ws.dcoef) {
+            if (int r = dalloc(d, ws.partial, size_t(d.total_chunk_coefs) * 128, err)) return r;
+            if (int r = dalloc(d, ws.dcoef, size_t(d.natomcoeffs) * d.nCols * 128, err)) return r;
+        }
+        if (d.nAtomImages > 0 && d.nchunks > 0 && d.max_atom_nc > 64 && !d.cscratch) {
+            if (int r = dalloc(d, d.cscratch, size_t(d.ncoeffs) * d.nCols * 128, err)) return r;
+        }
+        if (!ws.zero) {
+            if (int r = dalloc(d, ws.zero, NBLK, err)) return r;
+            HIP_TRY(hipMemsetAsync(ws.zero, 0, NBLK * sizeof(real_t), stream));
+        }
     }
     if ((parts & ACTION_PROJECT) && d.nAtomImages > 0 && d.nchunks > 0) {
         for (int L = 0; L <= 7; ++L) {
@@ -1297,20 +1763,31 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
 #define LAUNCH_PRJ(LL) hipLaunchKernelGGL((k_sho_project<real_t, LL>), dim3(n), dim3(128), 0, stream, ws.partial, x, \
                 d.chunks, bucket, d.chunk_coef, d.prj_blocks, d.rowindx, d.row_pos, d.image_pos, \
                 s.hgrid[0], s.hgrid[1], s.hgrid[2], s.hgrid[3], d.col_active, d.htab, d.xtab, d.prj_slot)
-            switch (L) {
-                case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
-                case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;
-                case 4: LAUNCH_PRJ(4); break; case 5: LAUNCH_PRJ(5); break;
-                case 6: LAUNCH_PRJ(6); break; default: LAUNCH_PRJ(7); break;
+#define LAUNCH_MFMA(LL) hipLaunchKernelGGL((k_sho_project_mfma<real_t, LL>), dim3(n), dim3(256), 0, stream, ws.partial, \
+                x, d.chunks, bucket, d.chunk_coef, d.prj_blocks, s.hgrid[3], d.col_active, d.htab, d.xtab, d.prj_slot)
+            if (d.prj_engine[sizeof(real_t) == 8 ? 0 : 1] == 1 && L <= 5) {   // the dense form on the matrix cores
+                switch (L) {
+                    case 0: LAUNCH_MFMA(0); break; case 1: LAUNCH_MFMA(1); break;
+                    case 2: LAUNCH_MFMA(2); break; case 3: LAUNCH_MFMA(3); break;
+                    case 4: LAUNCH_MFMA(4); break; default: LAUNCH_MFMA(5); break;
+                }
+            } else {
+                switch (L) {
+                    case 0: LAUNCH_PRJ(0); break; case 1: LAUNCH_PRJ(1); break;
+                    case 2: LAUNCH_PRJ(2); break; case 3: LAUNCH_PRJ(3); break;
+                    case 4: LAUNCH_PRJ(4); break; case 5: LAUNCH_PRJ(5); break;
+                    case 6: LAUNCH_PRJ(6); break; default: LAUNCH_PRJ(7); break;
+                }
             }
 #undef LAUNCH_PRJ
+#undef LAUNCH_MFMA
             HIP_TRY(hipGetLastError());
         }
         uint32_t const ncl = std::min<uint32_t>(d.max_atom_nc, 64);   // larger atoms use the global scratch
         size_t const lds = size_t(2) * ncl * 64 * sizeof(double);
         hipLaunchKernelGGL((k_sho_atoms<real_t>), dim3(d.nAtoms, d.nCols), dim3(64 * ATOM_WAVES), lds, stream, ws.dcoef,
                            ws.partial, d.chunk_coef, d.pair_chunk, d.sum_start, d.sum_index, d.image_starts,
-                           d.atom_lmax, d.atom_hs, d.atom_mat_off, d.image_phase, d.nCols, d.cscratch,
+                           d.atom_lmax, d.atom_hs, d.atom_mat_off, d.image_phase, d.nCols, cscratch,
                            d.atom_starts, d.E_col, d.col_active);
         HIP_TRY(hipGetLastError());
     }
@@ -1366,8 +1843,11 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     return 0;
 }
 
-template int device_action<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, std::string&, int, double*);
-template int device_action<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, std::string&, int, double*);
+template int device_action<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t, std::string&,
+                                   int, double*, Workspace<double>*, double*);
+template int device_action<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, std::string&,
+                                  int, double*, Workspace<float>*, double*);
+
 
 // Times every k_local tile shape on the caller's vectors (the whole action, reps launches each, HIP events on
 // `stream`) and keeps the fastest for this precision.  y is overwritten.
@@ -1375,8 +1855,9 @@ template <typename real_t>
 int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream, int reps,
                     double* best_ms, std::string& err) {
     int const v = sizeof(real_t) == 8 ? 0 : 1;
-    static int const SHAPES[8][3] = {{0, 16, 1}, {0, 16, 2}, {0, 16, 4}, {0, 32, 1}, {0, 32, 2}, {0, 32, 4},
-                                     {1, 32, 0}, {1, 64, 0}};
+    static int const SHAPES[14][3] = {{0, 16, 1}, {0, 16, 2}, {0, 16, 4}, {0, 32, 1}, {0, 32, 2}, {0, 32, 4},
+                                      {1, 32, 0}, {1, 64, 0}, {2, 16, 3}, {2, 32, 3}, {2, 64, 3}, {2, 16, 2},
+                                      {2, 32, 2}, {2, 64, 2}};
     double best = 1e300;
     int bkind = d.tune_kind[v], bjw = d.tune_jw[v], blb = d.tune_lb[v];
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1392,6 +1873,7 @@ int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t con
         HIP_TRY(hipEventCreate(&e1));
         for (auto const& sh : SHAPES) {
             int const kind = sh[0], jw = sh[1], lb = sh[2];
+            if (kind == 2 && jw == 64 && sizeof(real_t) == 8) continue;   // no FP64 kind-2 shape of 64 columns
             d.tune_kind[v] = kind; d.tune_jw[v] = jw; d.tune_lb[v] = lb;
             if (int r = device_action<real_t>(d, s, y, x, stream, err)) return r;  // warm-up
             HIP_TRY(hipEventRecord(e0, stream));
@@ -1407,6 +1889,29 @@ int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t con
         return 0;
     };
     if (int r = timed()) return finish(r);
+    // the projection's engine with the fastest shape: the vector form, then the matrix cores
+    if (TUNE_PROJECTION_ENGINE && d.nAtomImages > 0 && d.nchunks > 0) {
+        d.tune_kind[v] = bkind; d.tune_jw[v] = bjw; d.tune_lb[v] = blb;
+        double tbest[2] = {1e300, 1e300};
+        int const keep = d.prj_engine[v];
+        for (int eng = 0; eng < 2; ++eng) {
+            d.prj_engine[v] = eng;
+            if (int r = device_action<real_t>(d, s, y, x, stream, err)) { d.prj_engine[v] = keep; return finish(r); }
+            if (hipEventRecord(e0, stream) != hipSuccess) { d.prj_engine[v] = keep; return finish(2); }
+            for (int k = 0; k < reps; ++k)
+                if (int r = device_action<real_t>(d, s, y, x, stream, err)) { d.prj_engine[v] = keep; return finish(r); }
+            float ms = 0;
+            if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                d.prj_engine[v] = keep;
+                err = "autotune: event timing failed";
+                return finish(2);
+            }
+            tbest[eng] = double(ms) / std::max(1, reps);
+        }
+        d.prj_engine[v] = tbest[1] < tbest[0] ? 1 : 0;
+        best = std::min(tbest[0], tbest[1]);
+    }
     d.tuned[v] = true;
     if (best_ms) *best_ms = best;
     return finish(0);

@@ -3,6 +3,7 @@
 #include "../../include/gfa.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -19,6 +20,7 @@ struct gfa_plan {
     bool values_dirty = true;
     bool energy_set = false;
     int echo = 0;
+    uint16_t const* verified_colindex = nullptr;   // a caller's device colIndex found equal to the plan's colindx
 };
 
 namespace {
@@ -456,13 +458,37 @@ int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int l
             return fail(GFA_ERR_ARGUMENT, "tile shapes of kind 0: 16 or 32 columns, 1, 2 or 4 lines");
         if (kind == 1 && columns != 32 && columns != 64)
             return fail(GFA_ERR_ARGUMENT, "tile shapes of kind 1: 32 or 64 columns");
-        if (kind != 0 && kind != 1) return fail(GFA_ERR_ARGUMENT, "tile shape kind must be 0 or 1");
+        if (kind == 2 && ((columns != 16 && columns != 32 && !(columns == 64 && fp_bits == 32)) ||
+                          (lines != 2 && lines != 3)))
+            return fail(GFA_ERR_ARGUMENT, "tile shapes of kind 2: 16 or 32 columns (64 in FP32), 2 or 3 directions "
+                                          "prefetched");
+        if (kind < 0 || kind > 2) return fail(GFA_ERR_ARGUMENT, "tile shape kind must be 0, 1 or 2");
         int const v = fp_bits == 64 ? 0 : 1;
         plan->dev.tune_kind[v] = kind;
         plan->dev.tune_jw[v] = columns;
         plan->dev.tune_lb[v] = lines;
         plan->dev.tuned[v] = true;
         gfa::solver_release(plan);  // a captured solver graph holds the previous shape
+        return GFA_OK;
+    });
+}
+
+int gfa_set_projection_engine(gfa_plan* plan, int fp_bits, int engine) {
+    return guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (engine != 0 && engine != 1) return fail(GFA_ERR_ARGUMENT, "projection engine must be 0 (vector) or 1 (matrix)");
+        plan->dev.prj_engine[fp_bits == 64 ? 0 : 1] = engine;
+        gfa::solver_release(plan);  // a captured solver graph holds the previous launches
+        return GFA_OK;
+    });
+}
+
+int gfa_get_projection_engine(gfa_plan* plan, int fp_bits, int* engine) {
+    return guarded([&]() -> int {
+        if (!plan || !engine) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        *engine = plan->dev.prj_engine[fp_bits == 64 ? 0 : 1];
         return GFA_OK;
     });
 }
@@ -497,14 +523,65 @@ int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stre
 int gfa_action_multiply(gfa_plan* plan, int fp_bits, void* y, const void* x, const uint16_t* colIndex,
                         uint32_t nnzb, uint32_t nCols, uint32_t l2nX, void* stream, int precondition,
                         double* nflops) {
-    (void)colIndex; (void)l2nX;
-    if (!plan) { g_err = "null plan"; return GFA_ERR_ARGUMENT; }
-    if (nnzb != plan->host.nnzb || nCols != plan->host.nCols) {
-        g_err = "nnzb/nCols do not match the plan";
-        return GFA_ERR_ARGUMENT;
-    }
-    if (precondition) { g_err = "the action has no preconditioner"; return GFA_ERR_ARGUMENT; }
+    (void)l2nX;   // sizes tfQMRgpu's binary reductions; the library's reductions need no such level count
+    int const s = guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        if (nnzb != plan->host.nnzb || nCols != plan->host.nCols)
+            return fail(GFA_ERR_ARGUMENT, "nnzb/nCols do not match the plan");
+        if (precondition) return fail(GFA_ERR_ARGUMENT, "the action has no preconditioner");
+        // the blocks' columns come from the plan; a caller's colIndex (device memory) must hold the same values.
+        // Checked once per pointer (a blocking copy), so that tfQMRgpu's repeated calls with its own copy cost nothing
+        if (colIndex && colIndex != plan->verified_colindex) {
+            if (int e = ensure_device(plan)) return e;
+            if (colIndex != plan->dev.colindx) {
+                std::vector<uint16_t> h(nnzb);
+                hipError_t const e = hipMemcpy(h.data(), colIndex, size_t(nnzb) * sizeof(uint16_t), hipMemcpyDefault);
+                if (e != hipSuccess)
+                    return fail(GFA_ERR_ARGUMENT, std::string("colIndex is not readable device memory: ") +
+                                                      hipGetErrorString(e));
+                if (!std::equal(h.begin(), h.end(), plan->host.colindx.begin()))
+                    return fail(GFA_ERR_ARGUMENT, "colIndex differs from the plan's colindx");
+            }
+            plan->verified_colindex = colIndex;
+        }
+        return GFA_OK;
+    });
+    if (s != GFA_OK) return s;
     return gfa_action(plan, fp_bits, y, x, stream, nflops);
+}
+
+int gfa_action_benchmark(gfa_plan* plan, int fp_bits, int32_t iterations, double* seconds, double* nflops) {
+    return guarded([&]() -> int {
+        if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (iterations < 1) return fail(GFA_ERR_ARGUMENT, "iterations must be positive");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energy before applying the action");
+        if (int s = ensure_device(plan)) return s;
+        size_t const bytes = size_t(plan->host.nnzb) * 2 * 64 * 64 * (fp_bits / 8);
+        void* v[2] = {nullptr, nullptr};
+        auto release = [&](int status) {
+            for (void* q : v) if (q) (void)hipFree(q);
+            return status;
+        };
+        for (auto& q : v)
+            if (hipMalloc(&q, bytes) != hipSuccess) { q = nullptr; return release(fail(GFA_ERR_MEMORY, "benchmark vectors")); }
+        if (hipMemset(v[0], 0, bytes) != hipSuccess) return release(fail(GFA_ERR_DEVICE, "benchmark vectors"));
+        std::string err;
+        if (int s = gfa::capi_action(plan, fp_bits, v[1], v[0], nullptr, err, 0))   // workspace only
+            return release(fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err));
+        if (hipDeviceSynchronize() != hipSuccess) return release(fail(GFA_ERR_DEVICE, "benchmark set-up"));
+        auto const t0 = std::chrono::steady_clock::now();
+        for (int32_t it = 0; it < iterations; ++it) {   // y = A x, then swap, as the reference's loop
+            if (int s = gfa::capi_action(plan, fp_bits, v[1], v[0], nullptr, err))
+                return release(fail(s == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err));
+            if (hipDeviceSynchronize() != hipSuccess) return release(fail(GFA_ERR_DEVICE, "benchmark action"));
+            std::swap(v[0], v[1]);
+        }
+        double const dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (seconds) *seconds = dt;
+        if (nflops) *nflops = gfa::reference_flops(plan->host, fp_bits) * iterations;
+        return release(GFA_OK);
+    });
 }
 
 int gfa_solver_release(gfa_plan* plan) {

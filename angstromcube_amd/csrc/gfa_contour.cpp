@@ -201,6 +201,44 @@ int fail(int code, std::string const& msg) {
     return code;
 }
 
+// the source blocks of a plan description (its own box or explicit list) as block coordinates [nsrc][3]
+int source_blocks(gfa_plan_desc const* desc, std::vector<int32_t>& src) {
+    gfa_plan* base = nullptr;
+    if (int s = gfa_plan_create(desc, &base)) return s;
+    gfa_sizes sz;
+    gfa_plan_query(base, &sz);
+    uint32_t const nsrc = sz.nCols;
+    std::vector<float> colpos(size_t(nsrc) * 4);
+    int const s0 = gfa_plan_export(base, GFA_EXPORT_COL_POS, colpos.data(), colpos.size() * sizeof(float));
+    gfa_plan_destroy(base);
+    if (s0) return s0;
+    src.assign(size_t(nsrc) * 3, 0);
+    for (uint32_t c = 0; c < nsrc; ++c)   // the export is centred: block coordinate - nb / 2
+        for (int d = 0; d < 3; ++d)
+            src[3 * c + d] = int32_t(std::lround(double(colpos[4 * c + d]) + 0.5 * (desc->ng[d] / 4)));
+    return GFA_OK;
+}
+
+// a plan of desc with the given source blocks, the potential and the raw atom matrices set
+int plan_with_sources(gfa_plan_desc const* desc, std::vector<int32_t> const& srcs, double const* V, double const* hs,
+                      gfa_plan** out) {
+    gfa_plan_desc d = *desc;
+    d.sources = srcs.data();
+    d.nsources_explicit = int32_t(srcs.size() / 3);
+    gfa_plan* p = nullptr;
+    if (int s = gfa_plan_create(&d, &p)) return s;
+    if (int s = gfa_set_potential(p, V)) { gfa_plan_destroy(p); return s; }
+    size_t off = 0;
+    for (int32_t ia = 0; ia < desc->natoms; ++ia) {
+        int const L = int(desc->xyzZinso[8 * ia + 5]);
+        int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
+        if (int s = gfa_set_atom_matrices(p, ia, hs + off, nc)) { gfa_plan_destroy(p); return s; }
+        off += size_t(2) * nc * nc;
+    }
+    *out = p;
+    return GFA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -254,19 +292,9 @@ int gfa_contour_density(const gfa_plan_desc* desc, const double* V, const double
         auto const t0 = std::chrono::steady_clock::now();
 
         // the source blocks of the caller's plan description (its own box or explicit list)
-        gfa_plan* base = nullptr;
-        if (int s = gfa_plan_create(desc, &base)) return s;
-        gfa_sizes sz;
-        gfa_plan_query(base, &sz);
-        uint32_t const nsrc = sz.nCols;
-        std::vector<float> colpos(size_t(nsrc) * 4);
-        int s0 = gfa_plan_export(base, GFA_EXPORT_COL_POS, colpos.data(), colpos.size() * sizeof(float));
-        gfa_plan_destroy(base);
-        if (s0) return s0;
-        std::vector<int32_t> src(size_t(nsrc) * 3);
-        for (uint32_t c = 0; c < nsrc; ++c)   // the export is centred: block coordinate - nb / 2
-            for (int d = 0; d < 3; ++d)
-                src[3 * c + d] = int32_t(std::lround(double(colpos[4 * c + d]) + 0.5 * (desc->ng[d] / 4)));
+        std::vector<int32_t> src;
+        if (int s = source_blocks(desc, src)) return s;
+        uint32_t const nsrc = uint32_t(src.size() / 3);
 
         int const nE = opts->nE, nk = opts->nk;
         int const per = std::max(1, int((opts->max_groups > 0 ? opts->max_groups : 64) / int(nsrc)));   // energies per solve
@@ -287,20 +315,9 @@ int gfa_contour_density(const gfa_plan_desc* desc, const double* V, const double
                     E[2 * (size_t(i) * nsrc + c)] = opts->E[2 * (e0 + i)] + opts->E_Fermi;
                     E[2 * (size_t(i) * nsrc + c) + 1] = opts->E[2 * (e0 + i) + 1];
                 }
-            gfa_plan_desc d = *desc;
-            d.sources = srcs.data();
-            d.nsources_explicit = int32_t(ne * nsrc);
             gfa_plan* p = nullptr;
-            if (int s = gfa_plan_create(&d, &p)) return s;
+            if (int s = plan_with_sources(desc, srcs, V, hs, &p)) return s;
             auto done = [&](int s) { gfa_plan_destroy(p); return s; };
-            if (int s = gfa_set_potential(p, V)) return done(s);
-            size_t off = 0;
-            for (int32_t ia = 0; ia < desc->natoms; ++ia) {
-                int const L = int(desc->xyzZinso[8 * ia + 5]);
-                int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
-                if (int s = gfa_set_atom_matrices(p, ia, hs + off, nc)) return done(s);
-                off += size_t(2) * nc * nc;
-            }
             if (int s = gfa_set_energies(p, E.data(), dVol)) return done(s);
             std::vector<double> rk(size_t(ne) * nsrc * 128);
             for (int ik = 0; ik < nk; ++ik) {
@@ -339,6 +356,61 @@ int gfa_contour_density(const gfa_plan_desc* desc, const double* V, const double
         st.max_residual = max_res;
         st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         st.sources = int32_t(nsrc);
+        if (stats) *stats = st;
+        return GFA_OK;
+    });
+}
+
+int gfa_solve_chunked(const gfa_plan_desc* desc, const double* V, const double* hs, double E_re, double E_im,
+                      const double k[3], const gfa_solve_opts* opts, uint64_t max_bytes, double* rho,
+                      gfa_chunked_stats* stats) {
+    return guarded([&]() -> int {
+        if (!desc || !V || !opts || !rho) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (desc->source_radii) return fail(GFA_ERR_ARGUMENT, "the chunked solve uses one truncation radius (source_radii must be NULL)");
+        if (desc->natoms > 0 && !hs) return fail(GFA_ERR_ARGUMENT, "null atom matrices");
+        auto const t0 = std::chrono::steady_clock::now();
+        std::vector<int32_t> src;
+        if (int s = source_blocks(desc, src)) return s;
+        uint32_t const nsrc = uint32_t(src.size() / 3);
+        double const dVol = desc->hg[0] * desc->hg[1] * desc->hg[2];
+        double const kv[3] = {k ? k[0] : 0.0, k ? k[1] : 0.0, k ? k[2] : 0.0};
+        // device bytes per source block: the plan of all sources (host side only) and its solve state
+        uint64_t per_source = 0;
+        {
+            gfa_plan* all = nullptr;
+            if (int s = plan_with_sources(desc, src, V, hs, &all)) return s;
+            uint64_t sb = 0, pb = 0;
+            int const s1 = gfa_solve_memory(all, opts, &sb, &pb);
+            gfa_plan_destroy(all);
+            if (s1) return s1;
+            per_source = (sb + pb + nsrc - 1) / std::max<uint32_t>(1, nsrc);
+        }
+        uint32_t const chunk = max_bytes == 0 ? nsrc
+                             : uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(nsrc, max_bytes / std::max<uint64_t>(1, per_source))));
+        gfa_chunked_stats st{};
+        st.sources = int32_t(nsrc);
+        st.sources_per_chunk = int32_t(chunk);
+        for (uint32_t c0 = 0; c0 < nsrc; c0 += chunk) {
+            uint32_t const n = std::min(chunk, nsrc - c0);
+            std::vector<int32_t> srcs(src.begin() + 3 * size_t(c0), src.begin() + 3 * size_t(c0 + n));
+            gfa_plan* p = nullptr;
+            if (int s = plan_with_sources(desc, srcs, V, hs, &p)) return s;
+            auto done = [&](int s) { gfa_plan_destroy(p); return s; };
+            if (int s = gfa_set_energy(p, E_re, E_im, dVol)) return done(s);
+            if (int s = gfa_set_kpoint(p, kv)) return done(s);
+            gfa_solve_stats ss;
+            if (int s = gfa_solve(p, opts, rho + size_t(c0) * 128, &ss)) return done(s);
+            gfa_sizes sz;
+            gfa_plan_query(p, &sz);
+            st.chunks += 1;
+            st.converged_columns += ss.converged_columns;
+            st.total_columns += ss.total_columns;
+            st.max_residual = std::max(st.max_residual, ss.max_residual);
+            st.iterations = std::max(st.iterations, ss.iterations);
+            st.largest_chunk_bytes = std::max<uint64_t>(st.largest_chunk_bytes, sz.device_bytes + sz.solver_bytes);
+            gfa_plan_destroy(p);
+        }
+        st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) *stats = st;
         return GFA_OK;
     });

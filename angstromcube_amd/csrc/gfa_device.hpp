@@ -52,6 +52,9 @@ struct DevicePlan {
     // shapes measured fastest on a 159,256-block plan (FP64 k_local 16 columns x 2 lines, FP32 k_plane 32 columns)
     int tune_kind[2] = {0, 1}, tune_jw[2] = {16, 32}, tune_lb[2] = {2, 0};
     bool tuned[2] = {false, false};
+    // SHO projection per precision: 0 the sum-factorised form on the vector ALUs, 1 the dense form on the matrix
+    // cores (atoms with numax <= 5; wider ones always take the vector form)
+    int prj_engine[2] = {0, 0};
 
     // index data
     uint16_t* colindx = nullptr;       // [nnzb]
@@ -126,9 +129,12 @@ void device_free(DevicePlan& d);
 // parts of the action: the SHO projection and atom-matrix multiply (coefficients into the workspace), and
 // the local kernel (potential, kinetic, add-back of the workspace coefficients) that writes y
 enum { ACTION_PROJECT = 1, ACTION_LOCAL = 2, ACTION_ALL = 3 };
+// ext: a workspace the caller provides (no buffer of the plan is allocated or used); ext_cscratch its atom-multiply
+// scratch for numax 6, 7 atoms
 template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
-                  std::string& err, int parts = ACTION_ALL, double* dot = nullptr);
+                  std::string& err, int parts = ACTION_ALL, double* dot = nullptr, Workspace<real_t>* ext = nullptr,
+                  double* ext_cscratch = nullptr);
 
 // flop count of one action as the reference counts it (include/green_action.hxx:154-175)
 template <typename real_t>

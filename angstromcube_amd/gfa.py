@@ -24,9 +24,12 @@ EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS",
 EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
-             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_action_part", "gfa_gradient",
-             "gfa_solve", "gfa_solver_release", "gfa_solve_memory", "gfa_plan_prepare", "gfa_atom_density_matrices", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_action_benchmark", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_set_projection_engine", "gfa_get_projection_engine", "gfa_action_part", "gfa_gradient",
+             "gfa_solve", "gfa_solver_release", "gfa_solve_memory", "gfa_plan_prepare", "gfa_atom_density_matrices", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density", "gfa_solve_chunked",
              "gfa_last_error"]
+
+
+GFA_OK, GFA_ERR_ARGUMENT, GFA_ERR_DEVICE, GFA_ERR_STATE, GFA_ERR_MEMORY = 0, 1, 2, 3, 4
 
 
 class GfaError(RuntimeError):
@@ -70,6 +73,15 @@ class ContourStats(C.Structure):
                 ("total_columns", C.c_int32), ("max_residual", C.c_double), ("seconds", C.c_double)]
 
 
+class ChunkedStats(C.Structure):
+    _fields_ = [("sources", C.c_int32), ("sources_per_chunk", C.c_int32), ("chunks", C.c_int32),
+                ("converged_columns", C.c_int32), ("total_columns", C.c_int32), ("iterations", C.c_int32),
+                ("max_residual", C.c_double), ("seconds", C.c_double), ("largest_chunk_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged_columns", C.c_int32),
                 ("total_columns", C.c_int32), ("max_residual", C.c_double), ("seconds", C.c_double),
@@ -103,9 +115,12 @@ def lib():
         L.gfa_action.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_double)]
         L.gfa_action_multiply.argtypes = [vp, C.c_int, vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int,
                                           C.POINTER(C.c_double)]
+        L.gfa_action_benchmark.argtypes = [vp, C.c_int, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gfa_autotune.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
         L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         ip = C.POINTER(C.c_int)
+        L.gfa_set_projection_engine.argtypes = [vp, C.c_int, C.c_int]
+        L.gfa_get_projection_engine.argtypes = [vp, C.c_int, ip]
         L.gfa_get_tile_shape.argtypes = [vp, C.c_int, ip, ip, ip, ip]
         L.gfa_action_part.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gfa_gradient.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -121,6 +136,8 @@ def lib():
         L.gfa_energy_mesh.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp,
                                       C.c_int32, C.POINTER(C.c_int32)]
         L.gfa_contour_density.argtypes = [vp, dp, dp, C.POINTER(ContourOpts), dp, dp, C.POINTER(ContourStats)]
+        L.gfa_solve_chunked.argtypes = [C.POINTER(PlanDesc), dp, dp, C.c_double, C.c_double, dp, C.POINTER(SolveOpts),
+                                        C.c_uint64, dp, C.POINTER(ChunkedStats)]
         L.gfa_last_error.restype = C.c_char_p
         for name in C_SYMBOLS:
             getattr(L, name).restype = C.c_int if name != "gfa_last_error" else C.c_char_p
@@ -360,6 +377,15 @@ class GreenPlan:
         _check(lib().gfa_get_tile_shape(self.h, fp_bits, *[C.byref(t) for t in v]))
         return tuple(t.value for t in v)
 
+    def set_projection_engine(self, fp_bits, engine):
+        """0: the SHO projection's sum-factorised form on the vector ALUs, 1: its dense form on the matrix cores."""
+        _check(lib().gfa_set_projection_engine(self.h, fp_bits, engine))
+
+    def projection_engine(self, fp_bits):
+        e = C.c_int()
+        _check(lib().gfa_get_projection_engine(self.h, fp_bits, C.byref(e)))
+        return e.value
+
     def prepare(self, fp_bits):
         """Upload the plan and allocate the action's workspace of fp_bits (later actions allocate nothing)."""
         _check(lib().gfa_plan_prepare(self.h, fp_bits))
@@ -410,3 +436,30 @@ class GreenPlan:
         d["column_residuals"] = res
         d["column_halfsteps"] = hst
         return rho[..., 0] + 1j * rho[..., 1], d
+
+
+def solve_chunked(system, r_trunc, E, k=(0.0, 0.0, 0.0), max_bytes=0, fd_range=(4, 4, 4), nsources=(1, 1, 1),
+                  sources=None, r_proj=6.0, fp_bits=64, tolerance=1e-9, max_iterations=2000, max_refinements=4,
+                  method="tfqmr", inner_tolerance=0.0, fp32_max_iterations=0):
+    """The Green function of every source block of a plan description at one energy and k-point, in chunks of
+    source blocks that fit max_bytes of device memory (gfa_solve_chunked; 0: one chunk).  Returns
+    (rho[nsrc, 64] complex, stats dict)."""
+    d, keep = plan_desc(system, r_trunc, fd_range, nsources, sources, r_proj)
+    o = SolveOpts()
+    o.fp_bits, o.max_iterations, o.tolerance, o.max_refinements = fp_bits, max_iterations, tolerance, max_refinements
+    o.method = {"tfqmr": 0, "cocg": 1}[method]
+    o.inner_tolerance, o.fp32_max_iterations = inner_tolerance, fp32_max_iterations
+    V = np.ascontiguousarray(system.V, dtype=np.float64)
+    hs = np.concatenate([np.asarray(m, dtype=np.float64).reshape(-1) for m in system.hs]) if system.natoms else \
+        np.zeros(1)
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    base = GreenPlan(system, r_trunc, fd_range, nsources, sources, r_proj)
+    nsrc = base.nCols
+    base.close()
+    rho = np.zeros((nsrc, 64, 2))
+    st = ChunkedStats()
+    _check(lib().gfa_solve_chunked(C.byref(d), _dptr(V), _dptr(hs), float(E.real), float(E.imag), _dptr(kk),
+                                   C.byref(o), int(max_bytes), _dptr(rho), C.byref(st)))
+    del keep
+    return rho[..., 0] + 1j * rho[..., 1], st.as_dict()
+

@@ -130,13 +130,24 @@ int gfa_set_energies(gfa_plan* plan, const double* E, double dVol);
 int gfa_set_kpoint(gfa_plan* plan, const double k[3]);
 
 /* ---- the action y = (H - E S) x on device pointers; replaces action_t::multiply
- *      (include/green_action.hxx:140-176).  colIndex may be NULL (the plan holds its own copy); nnzb and
- *      nCols must match the plan.  stream is a hipStream_t (NULL: default stream).  The call is
- *      asynchronous on that stream.  *nflops (may be NULL) receives the reference's flop count. ---- */
+ *      (include/green_action.hxx:140-176).  stream is a hipStream_t (NULL: default stream).  The call is
+ *      asynchronous on that stream.  *nflops (may be NULL) receives the reference's flop count.
+ *      gfa_action_multiply has the reference's argument list: nnzb and nCols must match the plan; colIndex
+ *      (device memory, as tfQMRgpu passes it) may be NULL, else it must hold the plan's colindx, which is
+ *      checked with one blocking copy the first time a pointer is seen (GFA_ERR_ARGUMENT if it differs; a
+ *      caller that rewrites the array behind the same pointer is not re-checked); l2nX is accepted and unused
+ *      (it sizes tfQMRgpu's binary reductions); precondition must be 0 (has_preconditioner() is false). ---- */
 int gfa_action(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, double* nflops);
 int gfa_action_multiply(gfa_plan* plan, int fp_bits, void* y, const void* x, const uint16_t* colIndex,
                         uint32_t nnzb, uint32_t nCols, uint32_t l2nX, void* stream, int precondition,
                         double* nflops);
+
+/* ---- the reference's action timing loop: action_t::solve with a negative iteration count and no tfQMRgpu
+ *      (include/green_action.hxx:243-281, called by test_action at src/green_action.cxx:139-156).  Starting from
+ *      x = 0 it applies y = A x `iterations` times, swapping x and y, with a device synchronisation after every
+ *      call; *seconds receives the wall time of the loop and *nflops the reference's flop count of all calls.
+ *      Allocates and frees its two vectors. ---- */
+int gfa_action_benchmark(gfa_plan* plan, int fp_bits, int32_t iterations, double* seconds, double* nflops);
 
 /* ---- one part of the action, for timing and tests: part 1 = SHO projection and atom-matrix multiply (into the
  *      plan's coefficient workspace), 2 = the local kernel (potential, kinetic and SHO add-back of the workspace
@@ -150,8 +161,16 @@ int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* 
  *      No counterpart in the reference, whose kernels have fixed launch shapes. ---- */
 int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, int reps, double* best_ms);
 /* pins the tile shape instead: kind 0 (LDS accumulator): columns per work-group 16 or 32, kinetic lines per
- * thread in flight 1, 2 or 4; kind 1 (register accumulator per z plane): columns 32 or 64, lines ignored */
+ * thread in flight 1, 2 or 4; kind 1 (register accumulator per z plane): columns 32 or 64, lines ignored;
+ * kind 2 (16-byte lanes, each thread owning one x-, y- and z-line): columns 16 or 32 (or 64 in FP32), lines = the
+ * directions whose neighbour lines are loaded before the first stencil, 2 or 3 */
 int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int lines);
+/* the engine of the SHO projection (c = P^T x over the blocks an atom image touches) per precision: 0 the
+ * sum-factorised form on the vector ALUs (the default), 1 the dense form on the matrix cores (16x16x4 MFMA of the
+ * precision, FP32 or FP64; atoms with numax <= 5, wider ones keep the vector form).  gfa_autotune times both with
+ * its fastest tile shape and keeps the faster.  No counterpart in the reference. */
+int gfa_set_projection_engine(gfa_plan* plan, int fp_bits, int engine);
+int gfa_get_projection_engine(gfa_plan* plan, int fp_bits, int* engine);
 /* the tile shape the action of fp_bits uses (the autotuner's choice, the pinned one or the default); *tuned = 1
  * when it was set by gfa_autotune or gfa_set_tile_shape */
 int gfa_get_tile_shape(gfa_plan* plan, int fp_bits, int* kind, int* columns, int* lines, int* tuned);
@@ -207,10 +226,33 @@ int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve
  *      largest -- for fp_bits 32 including the FP64 work vectors of a column group that falls back to FP64 -- and
  *      the bytes of the plan's device data (an estimate before the plan is first used on the device).
  *      gfa_plan_prepare uploads the plan and allocates the action's workspace of fp_bits, so that later actions
- *      of that precision allocate nothing (for timing, graph capture and gfa_action_noalloc). ---- */
+ *      of that precision allocate nothing (for timing and graph capture). ---- */
 int gfa_solver_release(gfa_plan* plan);
 int gfa_solve_memory(gfa_plan* plan, const gfa_solve_opts* opts, uint64_t* solver_bytes, uint64_t* plan_bytes);
 int gfa_plan_prepare(gfa_plan* plan, int fp_bits);
+
+/* ---- the Green function of many source blocks in chunks: replaces the reference's distribution of the source
+ *      columns over MPI ranks (src/green_function.cxx:255-354) by a loop over chunks of source blocks on one device.
+ *      For the sources of desc (its box or explicit list, one truncation radius) at energy E and k-point k, every
+ *      chunk is a plan of its own with V and the raw atom matrices hs (as gfa_contour_density takes them), solved
+ *      with opts; the chunk size is the largest number of sources whose plan and solve state (gfa_solve_memory of
+ *      the plan of all sources, per source) fit max_bytes (0: one chunk).  Column groups are independent, so the
+ *      result equals the solve of one plan of all sources.  rho[nsrc][64][2] = G_ii / pi in source order. ---- */
+typedef struct {
+    int32_t sources;            /* source blocks of the description                                            */
+    int32_t sources_per_chunk;
+    int32_t chunks;
+    int32_t converged_columns;
+    int32_t total_columns;
+    int32_t iterations;         /* the largest iteration count of a chunk                                      */
+    double max_residual;
+    double seconds;
+    uint64_t largest_chunk_bytes; /* device memory of the largest chunk's plan and solve state                 */
+} gfa_chunked_stats;
+
+int gfa_solve_chunked(const gfa_plan_desc* desc, const double* V, const double* hs, double E_re, double E_im,
+                      const double k[3], const gfa_solve_opts* opts, uint64_t max_bytes, double* rho,
+                      gfa_chunked_stats* stats);
 
 /* ---- adjoint gradient of L = Im sum_c w_c Tr[G_c], the trace over the diagonal block of column group c ----
  *  With dG = -A^-1 dA G:  dL = -Im sum_c w_c <lambda_c, dA G_c>,  lambda_c = (H - E_c S)^-H B_c.

@@ -1,5 +1,6 @@
-// Host-only use of the C++ drop-in header: a plan of the 8x8x8 periodic cell, its sizes, and the
-// action_t contract types; no device work (runs without a GPU).
+// Host-only use of the C++ drop-in header: a plan of the 8x8x8 periodic cell, its sizes and tfQMRgpu members, the
+// action_t contract types and the solver's action keys; no device work (runs without a GPU).
+#include <complex>
 #include <cstdio>
 #include <vector>
 
@@ -14,7 +15,18 @@ int main() {
     gfa::action_t<double> action(&plan);
     static_assert(gfa::action_t<double>::LM == 64, "block size");
     std::printf("nnzb %llu nRows %u nCols %u atoms %u images %u\n", (unsigned long long)plan.nnzb(),
-                plan.sizes().nRows, plan.nCols(), plan.sizes().nAtoms, plan.sizes().nAtomImages);
+                plan.nRows, plan.nCols, plan.sizes().nAtoms, plan.sizes().nAtomImages);
+    // the members tfQMRgpu reads (include/action_plan.hxx:37-61)
+    if (plan.colindx.size() != plan.nnzb() || plan.rowstart.size() != plan.nRows + 1u ||
+        plan.rowstart.back() != plan.nnzb() || plan.subset.size() != plan.nCols || plan.iterations_needed != -99)
+        return 3;
+    // the action keys (src/green_solver.cxx:31-51): non-collinear keys are refused with a status, no exit
+    std::complex<double> rho[64];
+    gfa::green_solver_t s64(&plan, 0, 0, 64, 1), s32(&plan, 0, 0, 32, 1), n64(&plan, 0, 0, 64, 2), n32(&plan, 0, 0, 32, 2);
+    std::printf("keys %d %d %d %d\n", s64.action_key(), s32.action_key(), n64.action_key(), n32.action_key());
+    if (n64.solve(rho, plan.nCols, 100) != GFA_ERR_ARGUMENT || n32.solve(rho, plan.nCols, 100) != GFA_ERR_ARGUMENT) return 4;
+    if (s64.solve(rho, plan.nCols, 0) != GFA_OK) return 5;   // zero iterations: construction check only
+    std::printf("noco2 refused\n");
     try {
         int8_t const bad_bc[3] = {7, 0, 0};
         gfa::plan_t bad(ng, bad_bc, hg, xyzZinso, 6.0);

@@ -1,0 +1,115 @@
+"""The batched contour workload: the 32^3-block cell (128^3 grid) with 128 atoms at numax 5, 32 contour energies
+from -1 Ha to E_F = 0 (Im E from 0.0063 Ha at both ends to 0.3 Ha), 4 seeded k-points and truncation radii drawn
+from {6, 8, 10, 12} Bohr, one source block per instance; the wrapped cell makes the operator complex symmetric, so
+the mixed solve runs FP32 COCG inside FP64 refinement, as the benchmark does.
+
+(1) A slice of k-point 0: both ends of the contour (the slowest solves) and one instance of each remaining radius,
+in one batched plan.  Every column's true residual, measured with the oracle's CPU action on a single-instance plan
+of the same instance, meets 1e-9, and every instance's density G_ii / pi equals its own plain FP64 tfQMR solve
+(a plan of its own, as the reference's contour loop solves one energy at a time) to 1e-6.
+(2) The whole batch of k-point 0 (32 instances, 2048 columns): every column converged with its FP64 residual (the
+library's FP64 action) at most 1e-9, and the densities equal the FP64 COCG solve of the same batch to 1e-6."""
+import numpy as np
+import pytest
+
+from angstromcube_amd import gfa as G
+from angstromcube_amd import synthetic as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SOURCE = (16, 16, 16)
+MIXED = dict(fp_bits=32, tolerance=1e-9, max_iterations=8000, max_refinements=6, inner_tolerance=1e-30,
+             fp32_max_iterations=3000, method="cocg")
+
+
+def batch_plan(sysm, inst):
+    p = G.GreenPlan(sysm, r_trunc=max(d["radius"] for d in inst), fd_range=(4, 4, 4), sources=[SOURCE] * len(inst),
+                    source_radii=[d["radius"] for d in inst])
+    p.set_inputs(np.array([d["E"] for d in inst]), inst[0]["k"])
+    return p
+
+
+def single_plan(sysm, d):
+    p = G.GreenPlan(sysm, r_trunc=d["radius"], fd_range=(4, 4, 4), sources=[SOURCE])
+    p.set_inputs(d["E"], d["k"])
+    return p
+
+
+def block_keys(plan):
+    """(column group, target block position) of every block: plans of the same instances share these keys."""
+    pos = plan.export("ROW_POS")[plan.export("ROWINDX")][:, :3]
+    return [(int(c), tuple(np.round(r, 3))) for c, r in zip(plan.export("COLINDX"), pos)]
+
+
+def true_residuals(plan, ora, X):
+    AX = ora.action(X)
+    R = -AX
+    for b in plan.export("SUBSET"):
+        R[b, 0] += np.eye(64)
+    return np.sqrt((R * R).sum(axis=(0, 1, 2)))
+
+
+def slice_instances():
+    b0 = S.contour_instances()[:32]
+    sel = [0, 31]   # both ends of the contour: Im E = 0.0063 Ha
+    for r in (6.0, 8.0, 10.0, 12.0):
+        if not any(b0[i]["radius"] == r for i in sel):
+            sel.append(next(i for i, d in enumerate(b0) if d["radius"] == r))
+    return [b0[i] for i in sel]
+
+
+def test_contour_slice_residuals_and_densities(device):
+    import torch
+    sysm = S.contour_32()
+    inst = slice_instances()
+    assert sorted({d["radius"] for d in inst}) == [6.0, 8.0, 10.0, 12.0]
+    assert min(d["E"].imag for d in inst) < 0.0065
+    plan = batch_plan(sysm, inst)
+    print(f"slice of {len(inst)} instances (radii {[d['radius'] for d in inst]}): {plan.nnzb} blocks", flush=True)
+    X = torch.empty((plan.nnzb, 2, 64, 64), dtype=torch.float64, device=device)
+    rho, st = plan.solve(X=X, **MIXED)
+    torch.cuda.synchronize()
+    assert st["converged_columns"] == st["total_columns"], st
+    Xh = X.cpu().numpy()
+    keys = block_keys(plan)
+    plan.close()
+    for c, d in enumerate(inst):
+        sp = single_plan(sysm, d)
+        # the batched plan's column group c, in the single plan's block order
+        index = {k[1]: b for b, k in enumerate(keys) if k[0] == c}
+        order = [index[k[1]] for k in block_keys(sp)]
+        assert len(order) == sp.nnzb
+        Xc = np.ascontiguousarray(Xh[order])
+        r = true_residuals(sp, O.Oracle(sp, d["E"], d["k"]), Xc)
+        X1 = torch.empty((sp.nnzb, 2, 64, 64), dtype=torch.float64, device=device)
+        rho1, s1 = sp.solve(fp_bits=64, tolerance=1e-9, max_iterations=8000, max_refinements=2, X=X1)
+        torch.cuda.synchronize()
+        drho = np.abs(rho[c] - rho1[0]).max() / np.abs(rho1[0]).max()
+        print(f"  instance E = {d['E']:.4f}, r = {d['radius']}: {sp.nnzb} blocks, oracle residual {r.max():.2e}, "
+              f"FP64 tfQMR {s1['iterations']} half-steps, |rho - rho_fp64| / |rho_fp64| = {drho:.2e}", flush=True)
+        assert r.max() <= 1e-9
+        assert s1["converged_columns"] == 64
+        assert drho <= 1e-6
+        sp.close()
+
+
+def test_contour_batch_of_32_instances(device):
+    import torch
+    sysm = S.contour_32()
+    inst = S.contour_instances()[:32]
+    plan = batch_plan(sysm, inst)
+    print(f"batch of 32 instances: {plan.nnzb} blocks", flush=True)
+    rho, st = plan.solve(**MIXED)
+    res = st["column_residuals"]
+    print(f"  mixed COCG: {st['converged_columns']}/{st['total_columns']} columns, {st['iterations']} applications, "
+          f"max residual {st['max_residual']:.2e}", flush=True)
+    assert st["converged_columns"] == st["total_columns"] == 2048
+    assert res.max() <= 1e-9 and st["max_residual"] <= 1e-9
+    plan.release_solver()
+    rho64, s64 = plan.solve(fp_bits=64, tolerance=1e-9, max_iterations=8000, max_refinements=2, method="cocg")
+    assert s64["converged_columns"] == 2048
+    d = np.abs(rho - rho64).max(axis=1) / np.abs(rho64).max(axis=1)
+    print(f"  FP64 COCG: {s64['iterations']} applications; worst instance |rho - rho_fp64| / |rho_fp64| = {d.max():.2e}")
+    assert d.max() <= 1e-6
+    plan.close()

@@ -21,3 +21,4 @@ def test_cxx_drop_in(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "nnzb 512" in out.stdout and "rejected" in out.stdout
+    assert "keys 64021 32021 64022 32022" in out.stdout and "noco2 refused" in out.stdout

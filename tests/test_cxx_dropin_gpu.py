@@ -1,0 +1,100 @@
+"""The C++ drop-in header on the device (tests/cxx/drop_in_gpu.cpp, built by __graft_entry__.build()).
+
+The program builds the plan of the 8^3-block periodic cell (4 atoms numax 3, Gamma, E = -0.5 + 0.05i Ha, one
+source block, radius 6 Bohr) through gfa::plan_t, applies gfa::action_t<double> and gfa::action_t<float>
+::multiply with the plan's colindx in device memory (the call tfQMRgpu makes, include/green_action.hxx:140-149),
+and solves with gfa::green_solver_t for the action keys 64021 and 32021 (src/green_solver.cxx:31-51).  Checked
+here: the actions against the oracle (1e-12 FP64, 2e-5 FP32), the densities against the FP64 solution the oracle's
+CPU TFQMR reaches with the same shadow residual, the refused keys 64022/32022 and a colIndex that differs from
+the plan's."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from angstromcube_amd import gfa as G
+from angstromcube_amd import synthetic as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cxx", "build", "drop_in_gpu")
+E = complex(-0.5, 0.05)
+K = (0.0, 0.0, 0.0)
+R_TRUNC = 6.0
+SEED = 0x5EEDA43C   # the library's default shadow seed: gfa_solve_opts.shadow_seed = 0 selects it
+
+
+def write_inputs(d, sysm, x):
+    with open(os.path.join(d, "desc.txt"), "w") as f:
+        f.write(" ".join(str(v) for v in (*sysm.ng, *sysm.bc, repr(sysm.h), repr(R_TRUNC), repr(E.real),
+                                          repr(E.imag), *K)) + "\n")
+    np.ascontiguousarray(sysm.xyzZinso, dtype=np.float64).tofile(os.path.join(d, "xyzZinso.bin"))
+    np.ascontiguousarray(sysm.V, dtype=np.float64).tofile(os.path.join(d, "V.bin"))
+    with open(os.path.join(d, "hs.bin"), "wb") as f:
+        for m in sysm.hs:
+            m = np.ascontiguousarray(m, dtype=np.float64)
+            f.write(np.int32(m.shape[-1]).tobytes())
+            f.write(m.tobytes())
+    np.ascontiguousarray(x, dtype=np.float64).tofile(os.path.join(d, "x.bin"))
+
+
+def parse_info(path):
+    lines = open(path).read().splitlines()
+    return lines, {ln.split()[0] + (" " + ln.split()[1] if ln.startswith("key") else ""): ln for ln in lines}
+
+
+def test_drop_in_action_and_solver_on_device(tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cxx/build/drop_in_gpu is not built: run __graft_entry__.build()")
+    sysm = S.small_periodic()
+    plan = G.GreenPlan(sysm, r_trunc=R_TRUNC)
+    plan.set_inputs(E, K)
+    x = np.random.default_rng(7).standard_normal((plan.nnzb, 2, 64, 64))
+    write_inputs(str(tmp_path), sysm, x)
+    print(f"drop_in_gpu on the 8^3-block periodic cell: nnzb {plan.nnzb}", flush=True)
+    out = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines, info = parse_info(os.path.join(tmp_path, "info.txt"))
+    print("\n".join(lines))
+
+    # plan members tfQMRgpu reads
+    assert f"nnzb {plan.nnzb} nRows {plan.nRows} nCols {plan.nCols}" in info["nnzb"]
+    assert f"rowstart_back {plan.nnzb}" in info["nnzb"]
+    assert f"subset0 {int(plan.export('SUBSET')[0])}" in info["nnzb"]
+
+    # the actions against the oracle
+    ora = O.Oracle(plan, E, K)
+    y_ref = ora.action(x)
+    y64 = np.fromfile(os.path.join(tmp_path, "y64.bin"), dtype=np.float64).reshape(x.shape)
+    y32 = np.fromfile(os.path.join(tmp_path, "y32.bin"), dtype=np.float32).reshape(x.shape)
+    e64 = np.abs(y64 - y_ref).max() / np.abs(y_ref).max()
+    y_ref32 = ora.action(x.astype(np.float32).astype(np.float64))
+    e32 = np.abs(y32 - y_ref32).max() / np.abs(y_ref32).max()
+    print(f"action_t<double>::multiply vs oracle {e64:.2e}, action_t<float> {e32:.2e}")
+    assert e64 <= 1e-12
+    assert e32 <= 2e-5
+
+    # refusals: a colIndex that is not the plan's, Noco = 2
+    assert info["bad_colindex"].endswith(f"status {G.GFA_ERR_ARGUMENT}")
+    assert info["noco2_multiply"].endswith(f"status {G.GFA_ERR_ARGUMENT}")
+    assert info["key 64022"].endswith(f"status {G.GFA_ERR_ARGUMENT}")
+    assert info["key 32022"].endswith(f"status {G.GFA_ERR_ARGUMENT}")
+    assert info["zero_iterations"].startswith("zero_iterations status 0 benchmark status 0")
+
+    # the solves: every column converged, densities equal to the oracle TFQMR's to the solve tolerance
+    for key in ("key 64021", "key 32021"):
+        f = info[key].split()
+        assert f[3] == "0", info[key]
+        assert f[5] == f[7], info[key]          # converged == total
+        assert float(f[9]) <= 1e-9, info[key]
+    Xo, _, m_o = ora.tfqmr(tol=1e-12, maxit=4000, seed=SEED)
+    sub = plan.export("SUBSET")
+    rho_o = np.stack([np.diag(Xo[b, 0]) + 1j * np.diag(Xo[b, 1]) for b in sub]).reshape(-1) / np.pi
+    for name in ("rho64", "rho32"):
+        rho = np.fromfile(os.path.join(tmp_path, name + ".bin"), dtype=np.complex128)
+        err = np.abs(rho - rho_o).max() / np.abs(rho_o).max()
+        print(f"{name} vs oracle TFQMR (tolerance 1e-12, {m_o} half-steps): {err:.2e}")
+        assert err <= 1e-7, (name, err)

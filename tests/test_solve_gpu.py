@@ -167,7 +167,8 @@ def test_cocg_rejects_complex_phases(device):
 
 
 @pytest.mark.parametrize("kind,columns,lines", [(0, 16, 1), (0, 16, 2), (0, 16, 4), (0, 32, 1), (0, 32, 2), (0, 32, 4),
-                                               (1, 32, 0), (1, 64, 0)])
+                                               (1, 32, 0), (1, 64, 0), (2, 16, 3), (2, 32, 3), (2, 64, 3),
+                                               (2, 16, 2), (2, 32, 2), (2, 64, 2)])
 def test_cocg_with_every_tile_shape(device, kind, columns, lines):
     """COCG takes (p, A p) from the local kernel's epilogue; with each tile shape the FP64 and the mixed solves
     reach the tolerance (oracle-measured residual) and give the tfQMR solution."""
@@ -177,6 +178,8 @@ def test_cocg_with_every_tile_shape(device, kind, columns, lines):
         X64, _, _ = run(plan, device, fp_bits=64, tolerance=1e-9, max_iterations=2000, max_refinements=4)
         ora = O.Oracle(plan, E, k)
         for fp in (64, 32):
+            if kind == 2 and columns == 64 and fp == 64:
+                continue   # no such FP64 shape
             plan.set_tile_shape(fp, kind, columns, lines)
             print(f"tile {kind}:{columns}x{lines} {name} cocg fp{fp}", flush=True)
             Xc, _, sc = run(plan, device, fp_bits=fp, tolerance=1e-9, max_iterations=2000, max_refinements=6,
