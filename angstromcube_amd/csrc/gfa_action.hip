@@ -30,9 +30,7 @@ namespace gfa {
 namespace {
 
 constexpr int NBLK = 2 * 64 * 64;  // elements of one block
-// the autotuner also times the two engines of the SHO projection (the matrix-core one is off until its parity
-// is established on the device)
-constexpr bool TUNE_PROJECTION_ENGINE = false;
+
 #ifndef LOCAL_TILE
 #define LOCAL_TILE 6   // xy tile edge (blocks) of the local kernel's processing order (5 to 7 measured best)
 #endif
@@ -407,6 +405,7 @@ struct LocalArgs {
     float const* xtab;
     uint32_t const* add_slot;  // slot of each add-list entry
     double* dot;             // if set: [nnzb][re, im][64] bilinear products sum_i x[b][i][j] y[b][i][j] per block
+    int add_local;           // 1: the local kernel adds the numax <= 5 images; 0: k_sho_add_mfma adds them after it
     real_t c8[3][5];         // 8th-order weights per direction
     real_t c16[3][9];        // 16th-order weights per direction
     int nh[3];               // neighbour blocks per side used in direction d (1 or 2)
@@ -656,7 +655,7 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
     // ---- SHO add-back of the atom images (numax <= 5) touching this block; the coefficients of
     //      each image are staged in the LDS buffer of the block, which is free after the kinetic passes ----
     uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
-    if (b1 > b0) {
+    if (a.add_local && b1 > b0) {
         uint32_t const col = a.colindx[blk];
         float const R2 = float(a.hgrid[3] * a.hgrid[3]);
         int const zq = lg;  // threads (jl, p, zq) own the z plane zq
@@ -862,7 +861,7 @@ __device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_
     constexpr int VE = 16 / sizeof(real_t), VPR = JW / VE;
     int const tid = threadIdx.x, j = slice * JW + jl;
     uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
-    if (b1 > b0) {
+    if (a.add_local && b1 > b0) {
         uint32_t const col = a.colindx[blk];
         float const R2 = float(a.hgrid[3] * a.hgrid[3]);
         for (uint32_t b = b0; b < b1; ++b) {
@@ -1237,7 +1236,7 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
 
     // ---- SHO add-back of the atom images (numax <= 5) touching the block; coefficients staged in ya ----
     uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
-    if (b1 > b0) {
+    if (a.add_local && b1 > b0) {
         uint32_t const col = a.colindx[blk];
         float const R2 = float(a.hgrid[3] * a.hgrid[3]);
         constexpr int VPR = JW / CPL;
@@ -1322,6 +1321,86 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
             a.dot[(blk * 2 + 1) * 64 + slice * JW + tid] = si;
         }
     }
+}
+
+// SHO add-back on the matrix cores (the dense form), after the local kernel: for every block touched by atom images
+// with numax <= 5, y[b][n][i] += sum_img sum_s P_img[i][s] D_img[s][n] (i the 64 points, n = plane * 64 + column,
+// s the coefficients padded to 64).  One work-group per block; 4 waves, wave w owns the columns n = 32 w .. 32 w + 31
+// and all four 16-point tiles.  P (the A operand, zero outside the sphere and beyond the coefficients) is built in
+// LDS per image; the coefficients D (the B operand) are read from global memory.  Operand layout as in
+// k_sho_project_mfma.
+template <typename real_t>
+__global__ __launch_bounds__(256) void k_sho_add_mfma(LocalArgs<real_t> a, uint32_t const* __restrict__ blocks,
+                                                      uint8_t const* __restrict__ col_active) {
+    using acc_t = typename Mfma<real_t>::acc_t;
+    constexpr int NCP = 64;
+    __shared__ real_t P[64 * NCP];   // [point][coefficient]
+    __shared__ real_t H[6][3][4];
+    __shared__ float xi2[3][4];
+    __shared__ uint16_t sxyz[NCP];   // coefficient s -> ix | iy << 4 | iz << 8 of the current image's numax
+    uint32_t const blk = blocks[blockIdx.x];
+    uint32_t const col = a.colindx[blk];
+    if (col_active && !col_active[col]) return;
+    int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6, kl = lane >> 4;
+    float const R2 = float(a.hgrid[3] * a.hgrid[3]);
+    acc_t acc[4][2];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[mt][t] = acc_t{0, 0, 0, 0};
+    int const nb0 = 32 * w + (lane & 15);   // B operand column of tile 0 (tile 1: + 16)
+    for (uint32_t b = a.add_start[blk]; b < a.add_start[blk + 1]; ++b) {
+        uint32_t const img = a.add_index[b];
+        int const L = a.image_lmax[img];
+        if (L > 5) continue;   // added by k_sho_add_wide
+        int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
+        __syncthreads();   // P of the previous image is no longer read
+        stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, 256);
+        if (tid == 0) {
+            int q = 0;
+            for (int iz = 0; iz <= L; ++iz)
+                for (int iy = 0; iy <= L - iz; ++iy)
+                    for (int ix = 0; ix <= L - iz - iy; ++ix) sxyz[q++] = uint16_t(ix | (iy << 4) | (iz << 8));
+            for (; q < NCP; ++q) sxyz[q] = 0xFFFF;
+        }
+        __syncthreads();
+        for (int e = tid; e < 64 * NCP; e += 256) {
+            int const i = e / NCP, c = e % NCP;
+            int const xx = i & 3, yy = (i >> 2) & 3, zz = i >> 4;
+            uint16_t const code = sxyz[c];
+            real_t v = 0;
+            if (code != 0xFFFF && sho_inside(xi2, R2, zz, yy, xx))
+                v = H[code & 15][0][xx] * H[(code >> 4) & 15][1][yy] * H[code >> 8][2][zz];
+            P[e] = v;
+        }
+        __syncthreads();
+        real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + nb0;
+        size_t const dstride = size_t(a.nCols) * 128;
+        int const nks = (nc + 3) / 4;
+        for (int ks = 0; ks < nks; ++ks) {
+            int const sk = ks * 4 + kl;
+            real_t const b0v = sk < nc ? dsrc[size_t(sk) * dstride] : real_t(0);
+            real_t const b1v = sk < nc ? dsrc[size_t(sk) * dstride + 16] : real_t(0);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                real_t const av = P[(mt * 16 + (lane & 15)) * NCP + ks * 4 + kl];
+                acc[mt][0] = Mfma<real_t>::mac(av, b0v, acc[mt][0]);
+                acc[mt][1] = Mfma<real_t>::mac(av, b1v, acc[mt][1]);
+            }
+        }
+    }
+    // y += acc: point mt * 16 + row, column n of tile t
+    real_t* yb = a.y + size_t(blk) * NBLK;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const i = mt * 16 + Mfma<real_t>::row(kl, r), n = nb0 + 16 * t;
+                real_t& o = yb[(n >> 6) * 4096 + i * 64 + (n & 63)];
+                o += acc[mt][t][r];
+            }
 }
 
 // SHO add-back of the atom images with numax 6 or 7, y += P d, on the blocks listed in blocks[]: one work-group of
@@ -1483,6 +1562,13 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
                 if (p.image_lmax[p.sho_add.index[k]] > 5) { wide.push_back(uint32_t(blk)); break; }
         d.n_wide_blocks = uint32_t(wide.size());
         if ((s = dupload(d, d.wide_blocks, wide, err))) return s;
+        // blocks touched by atom images with numax <= 5 (the add-back on the matrix cores)
+        std::vector<uint32_t> narrow;
+        for (uint64_t blk = 0; blk < p.nnzb; ++blk)
+            for (uint32_t k = p.sho_add.start[blk]; k < p.sho_add.start[blk + 1]; ++k)
+                if (p.image_lmax[p.sho_add.index[k]] <= 5) { narrow.push_back(uint32_t(blk)); break; }
+        d.n_narrow_blocks = uint32_t(narrow.size());
+        if ((s = dupload(d, d.narrow_blocks, narrow, err))) return s;
     }
     {   // (image, target row) slots of the Hermite table, for the add lists and the projection chunks below
         std::map<uint64_t, uint32_t> slot_of;
@@ -1739,7 +1825,6 @@ template <typename real_t>
 int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
                   std::string& err, int parts, double* dot, Workspace<real_t>* ext, double* ext_cscratch) {
     auto& ws = ext ? *ext : workspace<real_t>(d);
-    double* const cscratch = ext ? ext_cscratch : d.cscratch;
     if (!ext) {
         // every buffer of this precision is allocated on the first call, so that later calls (also under graph
         // capture) allocate nothing; parts = 0 only allocates
@@ -1754,6 +1839,12 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
             if (int r = dalloc(d, ws.zero, NBLK, err)) return r;
             HIP_TRY(hipMemsetAsync(ws.zero, 0, NBLK * sizeof(real_t), stream));
         }
+    }
+    // after the allocations above: the atom-multiply scratch of numax 6, 7 atoms exists from here on
+    double* const cscratch = ext ? ext_cscratch : d.cscratch;
+    if ((parts & ACTION_PROJECT) && d.nAtomImages > 0 && d.nchunks > 0 && d.max_atom_nc > 64 && !cscratch) {
+        err = "no scratch for the atom matrices of numax 6, 7 atoms";
+        return 1;
     }
     if ((parts & ACTION_PROJECT) && d.nAtomImages > 0 && d.nchunks > 0) {
         for (int L = 0; L <= 7; ++L) {
@@ -1803,6 +1894,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     a.E_col = d.E_col;
     a.htab = d.htab; a.xtab = d.xtab; a.add_slot = d.add_slot;
     a.dot = dot;
+    int const engine = d.prj_engine[sizeof(real_t) == 8 ? 0 : 1];
+    a.add_local = (engine == 1) ? 0 : 1;
     static int const c8i[5] = {-14350, 8064, -1008, 128, -9};
     static int const c16i[9] = {-924708642, 538137600, -94174080, 22830080, -5350800, 1053696, -156800, 15360, -735};
     double cdiag = 0;
@@ -1835,6 +1928,11 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
         else launch_local<real_t, 1, false>(KIND, JW, LB, g, stream, a);
     }
     HIP_TRY(hipGetLastError());
+    if (!a.add_local && d.n_narrow_blocks > 0) {   // the numax <= 5 images on the matrix cores
+        hipLaunchKernelGGL((k_sho_add_mfma<real_t>), dim3(d.n_narrow_blocks), dim3(256), 0, stream, a, d.narrow_blocks,
+                           d.col_active);
+        HIP_TRY(hipGetLastError());
+    }
     if (d.n_wide_blocks > 0) {
         hipLaunchKernelGGL((k_sho_add_wide<real_t>), dim3(d.n_wide_blocks), dim3(512), 0, stream, a, d.wide_blocks,
                            d.col_active);
@@ -1890,7 +1988,7 @@ int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t con
     };
     if (int r = timed()) return finish(r);
     // the projection's engine with the fastest shape: the vector form, then the matrix cores
-    if (TUNE_PROJECTION_ENGINE && d.nAtomImages > 0 && d.nchunks > 0) {
+    if (d.nAtomImages > 0 && d.nchunks > 0) {
         d.tune_kind[v] = bkind; d.tune_jw[v] = bjw; d.tune_lb[v] = blb;
         double tbest[2] = {1e300, 1e300};
         int const keep = d.prj_engine[v];

@@ -473,18 +473,18 @@ int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int l
     });
 }
 
-int gfa_set_projection_engine(gfa_plan* plan, int fp_bits, int engine) {
+int gfa_set_dyadic_engine(gfa_plan* plan, int fp_bits, int engine) {
     return guarded([&]() -> int {
         if (!plan) return fail(GFA_ERR_ARGUMENT, "null plan");
         if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
-        if (engine != 0 && engine != 1) return fail(GFA_ERR_ARGUMENT, "projection engine must be 0 (vector) or 1 (matrix)");
+        if (engine != 0 && engine != 1) return fail(GFA_ERR_ARGUMENT, "dyadic engine must be 0 (vector) or 1 (matrix)");
         plan->dev.prj_engine[fp_bits == 64 ? 0 : 1] = engine;
         gfa::solver_release(plan);  // a captured solver graph holds the previous launches
         return GFA_OK;
     });
 }
 
-int gfa_get_projection_engine(gfa_plan* plan, int fp_bits, int* engine) {
+int gfa_get_dyadic_engine(gfa_plan* plan, int fp_bits, int* engine) {
     return guarded([&]() -> int {
         if (!plan || !engine) return fail(GFA_ERR_ARGUMENT, "null argument");
         if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");

@@ -52,8 +52,9 @@ struct DevicePlan {
     // shapes measured fastest on a 159,256-block plan (FP64 k_local 16 columns x 2 lines, FP32 k_plane 32 columns)
     int tune_kind[2] = {0, 1}, tune_jw[2] = {16, 32}, tune_lb[2] = {2, 0};
     bool tuned[2] = {false, false};
-    // SHO projection per precision: 0 the sum-factorised form on the vector ALUs, 1 the dense form on the matrix
-    // cores (atoms with numax <= 5; wider ones always take the vector form)
+    // SHO projection and add-back per precision: 0 the sum-factorised forms on the vector ALUs (the add-back inside
+    // the local kernel), 1 the dense forms on the matrix cores (the add-back in its own kernel after the local one);
+    // atoms with numax > 5 always take the vector forms
     int prj_engine[2] = {0, 0};
 
     // index data
@@ -75,6 +76,8 @@ struct DevicePlan {
     uint32_t* add_index = nullptr;
     uint32_t* wide_blocks = nullptr;   // blocks touched by an atom image with numax > 5
     uint32_t n_wide_blocks = 0;
+    uint32_t* narrow_blocks = nullptr; // blocks touched by an atom image with numax <= 5
+    uint32_t n_narrow_blocks = 0;
     double* image_pos = nullptr;       // [nai][4]
     int8_t* image_lmax = nullptr;      // [nai]
     uint32_t* image_starts = nullptr;  // [nai+1]

@@ -4,6 +4,10 @@
 // This file must be compiled with -ffp-contract=off.
 #include "gfa_plan.hpp"
 
+#include <atomic>
+#include <cstdlib>
+#include <thread>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -14,6 +18,27 @@ namespace gfa {
 namespace {
 constexpr int X = 0, Y = 1, Z = 2;
 inline double sq(double x) { return x * x; }
+
+// f(i) for i in [0, n) on up to 16 host threads (GFA_PLAN_THREADS overrides), items handed out one at a time; the
+// callers write per-item results and merge them afterwards in item order, so the plan does not depend on the
+// thread count
+template <typename F>
+void parallel_for(size_t n, F&& f) {
+    unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (char const* e = std::getenv("GFA_PLAN_THREADS")) nt = unsigned(std::max(1, std::atoi(e)));
+    nt = unsigned(std::min<size_t>(nt, n));
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t)
+        pool.emplace_back([&]() {
+            for (size_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
+        });
+    for (auto& t : pool) t.join();
+}
 
 template <typename T>
 inline uint64_t idx3(T const n[3], int64_t x, int64_t y, int64_t z) {
@@ -204,7 +229,11 @@ int Plan::build_bsr(PlanInput const& in, std::string& err,
     sparsity.assign(nCols, std::vector<uint8_t>(nbox3, 0));
     std::vector<std::vector<uint16_t>> cols_of_target(nbox3);
     std::vector<uint64_t> diag_i3(nCols, ~uint64_t(0));  // target box index of each column's own block
-    for (uint32_t c = 0; c < nCols; ++c) {
+    // the target blocks of every source inside its sphere, in (z, y, x) order, per column in parallel
+    std::vector<std::vector<uint64_t>> hits(nCols);
+    std::vector<int> col_err(nCols, 0);   // 1: a block hit twice, 2: the source outside its own sphere
+    parallel_for(nCols, [&](size_t cc) {
+        uint32_t const c = uint32_t(cc);
         int32_t const* s = &source_coords[3 * c];
         // this source's sphere (the reference has one radius for all; src/green_function.cxx:580-581, 540)
         double const rc = source_radius[c];
@@ -220,6 +249,7 @@ int Plan::build_bsr(PlanInput const& in, std::string& err,
             }
         }
         bool diag_hit = false;
+        auto& sp = sparsity[c];
         for (int32_t bz = bfirst[Z]; bz <= blast[Z]; ++bz)
         for (int32_t by = bfirst[Y]; by <= blast[Y]; ++by)
         for (int32_t bx = bfirst[X]; bx <= blast[X]; ++bx) {
@@ -242,13 +272,19 @@ int Plan::build_bsr(PlanInput const& in, std::string& err,
             if (nci > 0) {
                 int64_t const tx = s[X] + bx - min_target[X], ty = s[Y] + by - min_target[Y], tz = s[Z] + bz - min_target[Z];
                 uint64_t const i3 = idx3(num_target, tx, ty, tz);
-                if (sparsity[c][i3]) { err = "a target block was hit twice"; return 1; }
-                sparsity[c][i3] = 1;
-                cols_of_target[i3].push_back(uint16_t(c));
+                if (sp[i3]) { col_err[c] = 1; return; }
+                sp[i3] = 1;
+                hits[c].push_back(i3);
                 if (0 == bx && 0 == by && 0 == bz) { diag_i3[c] = i3; diag_hit = true; }
             }
         }
-        if (!diag_hit) { err = "a source block is not inside its own truncation sphere"; return 1; }
+        if (!diag_hit) col_err[c] = 2;
+    });
+    for (uint32_t c = 0; c < nCols; ++c) {
+        if (col_err[c] == 1) { err = "a target block was hit twice"; return 1; }
+        if (col_err[c] == 2) { err = "a source block is not inside its own truncation sphere"; return 1; }
+        for (uint64_t i3 : hits[c]) cols_of_target[i3].push_back(uint16_t(c));   // columns in source order
+        std::vector<uint64_t>().swap(hits[c]);
     }
 
     // BSR rows in z, y, x order of the target box, columns in source order
@@ -320,16 +356,22 @@ int Plan::build_kinetic(int dd, int fd_request, std::vector<std::vector<uint8_t>
     int32_t num[3] = {num_target[X], num_target[Y], num_target[Z]};
     int32_t const num_dd = num[dd];
     num[dd] = 1;
-    auto find_inz = [&](int32_t const idx[3], uint32_t c) -> int64_t {
+    auto find_inz = [&](int32_t const idx[3], uint32_t c) -> int64_t {   // the columns of a row are ascending
         int32_t const r = iRow_of_coords[idx3(num_target, idx[X], idx[Y], idx[Z])];
-        for (uint32_t inz = row_start[r]; inz < row_start[r + 1]; ++inz)
-            if (colindx[inz] == c) return inz;
-        return -1;
+        auto const b = colindx.begin() + row_start[r], e = colindx.begin() + row_start[r + 1];
+        auto const it = std::lower_bound(b, e, uint16_t(c));
+        return (it != e && *it == c) ? int64_t(it - colindx.begin()) : -1;
     };
-    kp.start.assign(1, 0);
-    kp.list.clear();
-    std::vector<int32_t> line;
-    for (uint32_t c = 0; c < nCols; ++c) {
+    // the lines of every column in parallel, then concatenated in column order
+    std::vector<std::vector<int32_t>> col_list(nCols);
+    std::vector<std::vector<uint32_t>> col_len(nCols);
+    std::vector<std::string> col_err(nCols);
+    parallel_for(nCols, [&](size_t cc) {
+        uint32_t const c = uint32_t(cc);
+        std::string& err = col_err[c];
+        std::vector<int32_t> line;
+        auto& out = col_list[c];
+        auto& lens = col_len[c];
         auto const& sp = sparsity[c];
         for (int32_t iz = 0; iz < num[Z]; ++iz)
         for (int32_t iy = 0; iy < num[Y]; ++iy)
@@ -341,7 +383,7 @@ int Plan::build_kinetic(int dd, int fd_request, std::vector<std::vector<uint8_t>
                 idx[dd] = id;
                 if (sp[idx3(num_target, idx[X], idx[Y], idx[Z])]) {
                     int64_t const inz = find_inz(idx, c);
-                    if (inz < 0) { err = "inconsistent sparsity pattern"; return 1; }
+                    if (inz < 0) { err = "inconsistent sparsity pattern"; return; }
                     if (first < 0) first = id;
                     last = id;
                     line.push_back(int32_t(inz) + CUBE_EXISTS);
@@ -351,7 +393,7 @@ int Plan::build_kinetic(int dd, int fd_request, std::vector<std::vector<uint8_t>
             if (len <= size_t(NHALO)) continue;
             for (int ih = 0; ih < NHALO; ++ih) line.push_back(CUBE_IS_ZERO);
             if (periodicity) {
-                if (uint32_t(last + 1 - first) != periodicity) { err = "periodic line does not span the cell"; return 1; }
+                if (uint32_t(last + 1 - first) != periodicity) { err = "periodic line does not span the cell"; return; }
                 for (int side = 0; side < 2; ++side) {
                     for (int ih = 0; ih < NHALO; ++ih) {
                         int32_t const j = side ? (last - int32_t(periodicity) + ih + 1)
@@ -366,9 +408,17 @@ int Plan::build_kinetic(int dd, int fd_request, std::vector<std::vector<uint8_t>
                     }
                 }
             }
-            kp.list.insert(kp.list.end(), line.begin(), line.end());
-            kp.start.push_back(uint32_t(kp.list.size()));
+            out.insert(out.end(), line.begin(), line.end());
+            lens.push_back(uint32_t(line.size()));
         }
+    });
+    kp.start.assign(1, 0);
+    kp.list.clear();
+    for (uint32_t c = 0; c < nCols; ++c) {
+        if (!col_err[c].empty()) { err = col_err[c]; return 1; }
+        kp.list.insert(kp.list.end(), col_list[c].begin(), col_list[c].end());
+        for (uint32_t n : col_len[c]) kp.start.push_back(kp.start.back() + n);
+        std::vector<int32_t>().swap(col_list[c]);
     }
     return 0;
 }
@@ -399,54 +449,68 @@ int Plan::build_dyadic(std::string& err) {
     std::vector<std::vector<uint32_t>> cubes;  // target rows touched by each contributing image
     std::vector<uint8_t> copy_contributes(ncopies_all, 0);
 
+    // every (periodic image shift, atom, copy) in the reference's loop order; the target rows each one touches
+    // (corner test of the projection sphere) are found in parallel and the touching ones kept in that order
+    struct Cand { int8_t ishift[3]; int32_t ia; size_t iaa; double pos[3]; };
+    std::vector<Cand> cand;
     for (int zi = -images[Z]; zi <= images[Z]; ++zi)
     for (int yi = -images[Y]; yi <= images[Y]; ++yi)
     for (int xi = -images[X]; xi <= images[X]; ++xi) {
         int const ishift[3] = {xi, yi, zi};
         size_t iaa = 0;
-        for (int32_t ia = 0; ia < natoms; ++ia) {
-            double const sigma = xyzZinso[ia * 8 + 6];
-            double const r_projection = r_proj * sigma;
-            double const r2projection = sq(r_projection);
-            double const r2projection_plus = sq(r_projection + r_block_circ);
+        for (int32_t ia = 0; ia < natoms; ++ia)
             for (int zc = -copies[Z]; zc <= copies[Z]; ++zc)
             for (int yc = -copies[Y]; yc <= copies[Y]; ++yc)
             for (int xc = -copies[X]; xc <= copies[X]; ++xc) {
                 int const cshift[3] = {xc, yc, zc};
-                double pos[3];
-                for (int d = 0; d < 3; ++d) pos[d] = xyzZinso[ia * 8 + d] + (ishift[d] + cshift[d]) * cell[d];
-                std::vector<uint32_t> hit;
-                for (uint32_t r = 0; r < nRows; ++r) {
-                    float const* rc = &row_pos[4 * r];
-                    double cd2 = 0;
-                    for (int d = 0; d < 3; ++d) {
-                        double const centre = (rc[d] * 4.f + 2.0) * hg[d];
-                        cd2 += sq(centre - pos[d]);
-                    }
-                    if (cd2 <= r2projection_plus) {
-                        double d2c[3][2];
-                        for (int d = 0; d < 3; ++d)
-                            for (int ii = 0; ii < 2; ++ii)
-                                d2c[d][ii] = sq((rc[d] * 4.0 + ii * 3 + 0.5) * hg[d] - pos[d]);
-                        int nci = 0;
-                        for (int i8 = 0; i8 < 8; ++i8) {
-                            double const d2 = d2c[X][i8 & 1] + d2c[Y][(i8 >> 1) & 1] + d2c[Z][i8 >> 2];
-                            nci += (d2 < r2projection);
-                        }
-                        if (nci > 0) hit.push_back(r);
-                    }
+                Cand k;
+                for (int d = 0; d < 3; ++d) {
+                    k.ishift[d] = int8_t(ishift[d]);
+                    k.pos[d] = xyzZinso[ia * 8 + d] + (ishift[d] + cshift[d]) * cell[d];
                 }
-                if (!hit.empty()) {
-                    Image im;
-                    for (int d = 0; d < 3; ++d) { im.pos[d] = pos[d]; im.shift[d] = int8_t(ishift[d]); }
-                    im.iaa = iaa; im.ia = ia;
-                    img_data.push_back(im);
-                    cubes.push_back(std::move(hit));
-                    copy_contributes[iaa] = 1;
+                k.ia = ia;
+                k.iaa = iaa++;
+                cand.push_back(k);
+            }
+    }
+    std::vector<std::vector<uint32_t>> cand_hits(cand.size());
+    parallel_for(cand.size(), [&](size_t ic) {
+        Cand const& k = cand[ic];
+        double const sigma = xyzZinso[k.ia * 8 + 6];
+        double const r_projection = r_proj * sigma;
+        double const r2projection = sq(r_projection);
+        double const r2projection_plus = sq(r_projection + r_block_circ);
+        double const* pos = k.pos;
+        auto& hit = cand_hits[ic];
+        for (uint32_t r = 0; r < nRows; ++r) {
+            float const* rc = &row_pos[4 * r];
+            double cd2 = 0;
+            for (int d = 0; d < 3; ++d) {
+                double const centre = (rc[d] * 4.f + 2.0) * hg[d];
+                cd2 += sq(centre - pos[d]);
+            }
+            if (cd2 <= r2projection_plus) {
+                double d2c[3][2];
+                for (int d = 0; d < 3; ++d)
+                    for (int ii = 0; ii < 2; ++ii)
+                        d2c[d][ii] = sq((rc[d] * 4.0 + ii * 3 + 0.5) * hg[d] - pos[d]);
+                int nci = 0;
+                for (int i8 = 0; i8 < 8; ++i8) {
+                    double const d2 = d2c[X][i8 & 1] + d2c[Y][(i8 >> 1) & 1] + d2c[Z][i8 >> 2];
+                    nci += (d2 < r2projection);
                 }
-                ++iaa;
+                if (nci > 0) hit.push_back(r);
             }
         }
+    });
+    for (size_t ic = 0; ic < cand.size(); ++ic) {
+        if (cand_hits[ic].empty()) continue;
+        Image im;
+        for (int d = 0; d < 3; ++d) { im.pos[d] = cand[ic].pos[d]; im.shift[d] = cand[ic].ishift[d]; }
+        im.iaa = cand[ic].iaa; im.ia = cand[ic].ia;
+        img_data.push_back(im);
+        cubes.push_back(std::move(cand_hits[ic]));
+        copy_contributes[cand[ic].iaa] = 1;
     }
     nAtomImages = uint32_t(img_data.size());
 

@@ -951,9 +951,13 @@ struct SolverState {
     // captured step
     hipGraphExec_t graph = nullptr;
     int graph_fp = 0;
+    int graph_steps = 0;                    // solver steps in the captured graph
     uint64_t graph_seed = 0;
     std::vector<void*> allocations;
     size_t held_bytes = 0;                  // device bytes of allocations[]
+    // per solve: operator applications after which a column group leaves (empty or 0: no cap), and counters
+    std::vector<int32_t> group_cap;
+    int host_syncs = 0, graph_captures = 0;
 };
 
 namespace {
@@ -1066,6 +1070,7 @@ int set_active(gfa_plan* plan, SolverState& st, std::vector<uint8_t> const& acti
         HIP_TRY(hipMemcpyAsync(st.act_order, st.h_act.data(), st.n_act_blocks * 4, hipMemcpyHostToDevice, st.stream));
     HIP_TRY(hipMemcpyAsync(st.col_active, active.data(), st.nCols, hipMemcpyHostToDevice, st.stream));
     HIP_TRY(hipStreamSynchronize(st.stream));  // the host vectors may change before the copies ran otherwise
+    ++st.host_syncs;
     drop_graph(st);
     return 0;
 }
@@ -1125,8 +1130,10 @@ int enqueue_step(gfa_plan* plan, SolverState& st, Vecs<T> const& V, uint64_t see
 }
 
 // Runs steps (each `halfsteps_per_step` half-steps = actions) until every column stopped or maxit half-steps:
-// replays a captured HIP graph of one step (eager: plain launches), polls the column status every
-// `steps_per_check` steps, retires finished column groups and re-captures for the smaller active set.
+// replays a captured HIP graph of the `steps_per_check` steps between two status polls (eager: plain launches),
+// polls the column status, and retires finished column groups.  A retired group's columns are frozen on the
+// device by their status, so the active set is compacted (and the graph captured again) only when the live blocks
+// fall below 7/8 of the captured set, or when a group reaches its iteration cap (its status stays active).
 struct NoHook { int operator()(int) { return 0; } };
 
 template <typename StepFn, typename HookFn = NoHook>
@@ -1140,11 +1147,14 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
             for (int k = 0; k < nsteps; ++k)
                 if (int r = step(s)) return r;
         } else {
-            if (!st.graph || st.graph_fp != fp || st.graph_seed != key) {
+            // a full poll interval is one graph of nsteps steps; a shorter last interval replays a one-step graph
+            int const gsteps = (nsteps == steps_per_check) ? nsteps : 1;
+            if (!st.graph || st.graph_fp != fp || st.graph_seed != key || st.graph_steps != gsteps) {
                 drop_graph(st);
                 hipGraph_t gr = nullptr;
                 HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                int const r = step(s);
+                int r = 0;
+                for (int k = 0; k < gsteps && !r; ++k) r = step(s);
                 hipError_t const e = hipStreamEndCapture(s, &gr);
                 if (r) { if (gr) (void)hipGraphDestroy(gr); return r; }
                 if (e != hipSuccess) { err = std::string("graph capture: ") + hipGetErrorString(e); return GFA_ERR_DEVICE; }
@@ -1153,8 +1163,10 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
                 if (e2 != hipSuccess) { st.graph = nullptr; err = std::string("graph instantiate: ") + hipGetErrorString(e2); return GFA_ERR_DEVICE; }
                 st.graph_fp = fp;
                 st.graph_seed = key;
+                st.graph_steps = gsteps;
+                ++st.graph_captures;
             }
-            for (int k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(st.graph, s));
+            for (int k = 0; k < nsteps / gsteps; ++k) HIP_TRY(hipGraphLaunch(st.graph, s));
         }
         m += halfsteps_per_step * nsteps;
         res.actions += halfsteps_per_step * nsteps;
@@ -1162,17 +1174,20 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
         if (int r = hook(m)) return r;   // e.g. a residual replacement, between graph replays
         HIP_TRY(hipMemcpyAsync(st.S_host, st.S + size_t(STATUS) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        ++st.host_syncs;
         std::vector<uint8_t> act = st.active;
-        bool any = false, changed = false;
+        bool any = false, changed = false, capped = false;
+        uint64_t live_blocks = 0;
         for (uint32_t c = 0; c < st.nCols; ++c) {
             if (!act[c]) continue;
             bool live = false;
             for (int j = 0; j < 64 && !live; ++j) live = (st.S_host[64 * c + j] == ACTIVE);
-            if (live) any = true;
+            if (live && !st.group_cap.empty() && st.group_cap[c] > 0 && m >= st.group_cap[c]) { live = false; capped = true; }
+            if (live) { any = true; live_blocks += st.h_cstart[c + 1] - st.h_cstart[c]; }
             else { act[c] = 0; changed = true; }
         }
         if (!any) break;
-        if (changed) {
+        if (changed && (capped || 8 * live_blocks < 7 * st.n_act_blocks)) {
             if (int r = set_active(plan, st, act, err)) return r;
             scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
         }
@@ -1181,6 +1196,7 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
         std::vector<double> ms(st.ncol);
         HIP_TRY(hipMemcpyAsync(ms.data(), st.S + size_t(MSTEP) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        ++st.host_syncs;
         for (uint32_t q = 0; q < st.ncol; ++q) st.col_halfsteps[q] += ms[q];
     }
     return 0;
@@ -1226,8 +1242,9 @@ int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p, T* q, hipStr
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
     dim3 const gb(st.nseg), tb(256), gs(st.nCols), ts(64 * FOLD_P);
     // (p, A p) from the action's epilogue when nothing is added to A p after the local kernel (no numax 6, 7
-    // atoms), else from a pass over p and q
-    bool const fused = capi_dev(plan).n_wide_blocks == 0;
+    // atoms, add-back on the vector ALUs), else from a pass over p and q
+    DevicePlan const& dv = capi_dev(plan);
+    bool const fused = dv.n_wide_blocks == 0 && dv.prj_engine[fp == 64 ? 0 : 1] == 0;
     if (int e = capi_action(plan, fp, q, p, s, err, ACTION_ALL, fused ? st.bpart : nullptr)) return e;
     if (fused) hipLaunchKernelGGL(k_fold_blocks, gb, dim3(64), 0, s, st.bpart, g);
     else hipLaunchKernelGGL((k_cocg_pq<T>), gb, tb, 0, s, p, q, g);
@@ -1400,6 +1417,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     auto fetch_resid = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(resid.data(), st.res, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        ++st.host_syncs;
         return 0;
     };
     auto max_resid = [&]() { double mx = 0; for (double v : resid) mx = std::max(mx, v); return mx; };
@@ -1414,6 +1432,10 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         std::vector<double> tabs(st.ncol, o.tolerance * 0.5);
         HIP_TRY(hipMemcpyAsync(st.tol_abs, tabs.data(), st.ncol * sizeof(double), hipMemcpyHostToDevice, s));
     }
+    st.host_syncs = 0;
+    st.graph_captures = 0;
+    if (o.group_max_iterations) st.group_cap.assign(o.group_max_iterations, o.group_max_iterations + st.nCols);
+    else st.group_cap.clear();
     std::vector<uint8_t> all(st.nCols, 1);
     st.col_halfsteps.assign(st.ncol, 0.0);
     if (int r = set_active(plan, st, all, err)) return r;
@@ -1535,6 +1557,10 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         HIP_TRY(hipMemcpyAsync(rho, out, size_t(st.ncol) * 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
+    ++st.host_syncs;
+    S.host_syncs = st.host_syncs;
+    S.graph_captures = st.graph_captures;
+    st.group_cap.clear();
     S.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     S.flops = double(S.actions) * reference_flops(capi_host(plan), o.fp_bits);
     if (stats) *stats = S;

@@ -24,7 +24,7 @@ EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS",
 EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
-             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_action_benchmark", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_set_projection_engine", "gfa_get_projection_engine", "gfa_action_part", "gfa_gradient",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_action_benchmark", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_set_dyadic_engine", "gfa_get_dyadic_engine", "gfa_action_part", "gfa_gradient",
              "gfa_solve", "gfa_solver_release", "gfa_solve_memory", "gfa_plan_prepare", "gfa_atom_density_matrices", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density", "gfa_solve_chunked",
              "gfa_last_error"]
 
@@ -59,7 +59,7 @@ class SolveOpts(C.Structure):
                 ("X_device", C.c_void_p), ("stream", C.c_void_p), ("inner_tolerance", C.c_double),
                 ("echo", C.c_int32), ("eager", C.c_int32), ("autotune", C.c_int32),
                 ("fp32_max_iterations", C.c_int32), ("residuals", C.POINTER(C.c_double)), ("halfsteps", C.POINTER(C.c_int32)),
-                ("method", C.c_int32), ("reserved", C.c_int32)]
+                ("method", C.c_int32), ("reserved", C.c_int32), ("group_max_iterations", C.POINTER(C.c_int32))]
 
 
 class ContourOpts(C.Structure):
@@ -85,7 +85,8 @@ class ChunkedStats(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged_columns", C.c_int32),
                 ("total_columns", C.c_int32), ("max_residual", C.c_double), ("seconds", C.c_double),
-                ("flops", C.c_double), ("actions", C.c_int32), ("reserved", C.c_int32)]
+                ("flops", C.c_double), ("actions", C.c_int32), ("host_syncs", C.c_int32), ("graph_captures", C.c_int32),
+                ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -119,8 +120,8 @@ def lib():
         L.gfa_autotune.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
         L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         ip = C.POINTER(C.c_int)
-        L.gfa_set_projection_engine.argtypes = [vp, C.c_int, C.c_int]
-        L.gfa_get_projection_engine.argtypes = [vp, C.c_int, ip]
+        L.gfa_set_dyadic_engine.argtypes = [vp, C.c_int, C.c_int]
+        L.gfa_get_dyadic_engine.argtypes = [vp, C.c_int, ip]
         L.gfa_get_tile_shape.argtypes = [vp, C.c_int, ip, ip, ip, ip]
         L.gfa_action_part.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gfa_gradient.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -377,13 +378,14 @@ class GreenPlan:
         _check(lib().gfa_get_tile_shape(self.h, fp_bits, *[C.byref(t) for t in v]))
         return tuple(t.value for t in v)
 
-    def set_projection_engine(self, fp_bits, engine):
-        """0: the SHO projection's sum-factorised form on the vector ALUs, 1: its dense form on the matrix cores."""
-        _check(lib().gfa_set_projection_engine(self.h, fp_bits, engine))
+    def set_dyadic_engine(self, fp_bits, engine):
+        """0: the SHO projection and add-back in their sum-factorised forms on the vector ALUs; 1: in their dense forms
+        on the matrix cores (the add-back in a kernel after the local one)."""
+        _check(lib().gfa_set_dyadic_engine(self.h, fp_bits, engine))
 
-    def projection_engine(self, fp_bits):
+    def dyadic_engine(self, fp_bits):
         e = C.c_int()
-        _check(lib().gfa_get_projection_engine(self.h, fp_bits, C.byref(e)))
+        _check(lib().gfa_get_dyadic_engine(self.h, fp_bits, C.byref(e)))
         return e.value
 
     def prepare(self, fp_bits):
@@ -406,7 +408,8 @@ class GreenPlan:
         return sb.value, pb.value
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
-              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0, method="tfqmr"):
+              shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0, method="tfqmr",
+              group_max_iterations=None):
         """Solve (H - E S) X = B; returns (rho[nCols, 64] complex, stats dict)."""
         import torch
         o = SolveOpts()
@@ -423,6 +426,10 @@ class GreenPlan:
         o.fp32_max_iterations = fp32_max_iterations
         o.method = {"tfqmr": 0, "cocg": 1}[method]
         o.X_device = C.c_void_p(X.data_ptr()) if X is not None else None
+        caps = None
+        if group_max_iterations is not None:
+            caps = np.ascontiguousarray(np.broadcast_to(np.asarray(group_max_iterations, dtype=np.int32), (self.nCols,)))
+            o.group_max_iterations = caps.ctypes.data_as(C.POINTER(C.c_int32))
         o.stream = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream)
         rho = np.zeros((self.nCols, 64, 2))
         res = np.zeros(64 * self.nCols)

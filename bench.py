@@ -33,7 +33,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BLOCK_BYTES = {64: 2 * 64 * 64 * 8, 32: 2 * 64 * 64 * 4}
 # FP32 local-kernel tile shapes (kind, columns, lines) timed by --mode action
-FP32_SHAPES = [(0, 16, 1), (0, 16, 2), (0, 16, 4), (0, 32, 1), (0, 32, 2), (0, 32, 4), (1, 32, 0), (1, 64, 0)]
+FP32_SHAPES = [(0, 16, 1), (0, 16, 2), (0, 16, 4), (0, 32, 1), (0, 32, 2), (0, 32, 4), (1, 32, 0), (1, 64, 0),
+               (2, 16, 3), (2, 32, 3), (2, 64, 3), (2, 16, 2), (2, 32, 2), (2, 64, 2)]
+FP64_SHAPES = [(0, 16, 2), (1, 32, 0), (2, 16, 3), (2, 32, 3), (2, 16, 2), (2, 32, 2)]
 # HBM bytes per launch of the FP32 local kernel per tile shape on this workload (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE of --mode action, scaled by a plain copy's counts); the command that made them is in the file
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "local_fp32_contour32_traffic.json")
@@ -60,9 +62,10 @@ def parse():
     ap.add_argument("--cpu-halfsteps", type=int, default=10, help="half-steps of the timed CPU TFQMR sample")
     ap.add_argument("--budget-s", type=float, default=420.0,
                     help="wall seconds after which the optional batch-0 comparisons are skipped")
-    ap.add_argument("--mode", choices=["bench", "action", "contour", "batch128"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "action", "contour", "batch128", "batch"], default="bench",
                     help="action: only apply the action a few times (for profiler runs); batch128: the 128 "
-                         "instances in one plan")
+                         "instances in one plan; batch: one mixed solve of the batch --batch (for profiler runs)")
+    ap.add_argument("--batch", type=int, default=0, help="k-point batch of --mode batch")
     return ap.parse_args()
 
 
@@ -145,6 +148,9 @@ def main():
     if args.mode == "batch128":
         batch128_mode(G, sysm, instances, opts)
         return
+    if args.mode == "batch":
+        batch_mode(G, sysm, instances, n, args, opts)
+        return
 
     # ---- warmup (the first one also tunes the tile shapes of both precisions) ----
     shapes = None
@@ -167,6 +173,7 @@ def main():
     errors = {}
     fp64_cocg = {}
     per_inst = []
+    per_batch_hs = {}
     for k in range(args.steps):
         b = k % NBATCH
         inst = batch(instances, k, n)
@@ -189,8 +196,13 @@ def main():
         log(f"# step {k} (k-point {b}, {p.nnzb} blocks): {st['converged_columns']}/{st['total_columns']} columns "
             f"in {dt:.2f} s, {st['iterations']} applications, max residual {st['max_residual']:.2e}, "
             f"solver {p.sizes.solver_bytes / 1e9:.1f} GB")
+        hs_inst = [int(h) for h in st["column_halfsteps"].reshape(-1, 64).max(axis=1)]
+        if k < NBATCH:
+            per_batch_hs[b] = hs_inst
+            log(f"#   operator applications per instance: {hs_inst}; host syncs {st['host_syncs']}, "
+                f"graph captures {st['graph_captures']}")
         if k == 0:
-            per_inst = [int(h) for h in st["column_halfsteps"].reshape(-1, 64).max(axis=1)]
+            per_inst = hs_inst
         if keep:
             # the FP64 solution of the same batch (FP64 COCG, batched), untimed for the mixed rate
             p.release_solver()
@@ -303,6 +315,7 @@ def main():
         "mixed_vs_fp64_max_rel_error": max(errors.values()) if errors else None,
         "mixed_vs_fp64_rel_error_by_batch": {str(b): v for b, v in sorted(errors.items())} or None,
         "mixed_half_steps_per_instance_batch0": per_inst,
+        "mixed_applications_per_instance_by_batch": {str(b): v for b, v in sorted(per_batch_hs.items())},
         "fp64": fp64 or None,
         "mixed_over_fp64": round(value / fp64["columns_per_s"], 3) if fp64 else None,
         "action": {"fp64_ms": round(t_act64 * 1e3, 3), "fp32_ms": round(t_act32 * 1e3, 3),
@@ -334,6 +347,17 @@ def action_mode(G, sysm, instances, n, args, stream, dev):
         p.set_tile_shape(32, *shape)
         t = time_action(p, 32, 2, args.action_reps, stream)
         log(f"# local fp32 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+    for shape in FP64_SHAPES:
+        p.set_tile_shape(64, *shape)
+        t = time_action(p, 64, 2, args.action_reps, stream)
+        log(f"# local fp64 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+    for fp in (32, 64):   # the dyadic term's engines: part 1 (projection and atom matrices), part 2 (local kernel
+        for eng in (0, 1):  # with the add-back inside it, or followed by the matrix-core add-back) and the whole action
+            p.set_dyadic_engine(fp, eng)
+            t1, t2, t3 = (time_action(p, fp, part, args.action_reps, stream) for part in (1, 2, 3))
+            log(f"# dyadic engine {('vector', 'matrix')[eng]} fp{fp}: projection {t1 * 1e3:.3f} ms, local + add-back "
+                f"{t2 * 1e3:.3f} ms, action {t3 * 1e3:.3f} ms")
+        p.set_dyadic_engine(fp, 0)
     # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
     x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
     y = torch.empty_like(x)
@@ -341,6 +365,28 @@ def action_mode(G, sysm, instances, n, args, stream, dev):
         y.copy_(x)
     torch.cuda.synchronize()
     log(f"# copy calibration: {x.numel() * 8} bytes read and written per copy")
+
+
+def batch_mode(G, sysm, instances, n, args, opts):
+    """One mixed solve of k-point batch --batch with the default tile shapes, and its per-instance statistics."""
+    import torch
+    p = batch_plan(G, sysm, batch(instances, args.batch, n))
+    p.prepare(32)
+    p.prepare(64)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st = p.solve(fp_bits=32, **opts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    hs = st["column_halfsteps"].reshape(-1, 64).max(axis=1)
+    inst = batch(instances, args.batch, n)
+    out = {"mode": "batch", "batch": args.batch, "nnzb": int(p.nnzb), "seconds": round(dt, 3),
+           "columns_per_s": round(st["converged_columns"] / dt, 3), "applications": int(st["iterations"]),
+           "host_syncs": int(st["host_syncs"]), "graph_captures": int(st["graph_captures"]),
+           "instances": [{"E": [round(d["E"].real, 4), round(d["E"].imag, 4)], "radius": d["radius"],
+                          "applications": int(h)} for d, h in zip(inst, hs)]}
+    p.close()
+    print(json.dumps(out))
 
 
 def batch128_mode(G, sysm, instances, opts):

@@ -165,12 +165,14 @@ int gfa_autotune(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stre
  * kind 2 (16-byte lanes, each thread owning one x-, y- and z-line): columns 16 or 32 (or 64 in FP32), lines = the
  * directions whose neighbour lines are loaded before the first stencil, 2 or 3 */
 int gfa_set_tile_shape(gfa_plan* plan, int fp_bits, int kind, int columns, int lines);
-/* the engine of the SHO projection (c = P^T x over the blocks an atom image touches) per precision: 0 the
- * sum-factorised form on the vector ALUs (the default), 1 the dense form on the matrix cores (16x16x4 MFMA of the
- * precision, FP32 or FP64; atoms with numax <= 5, wider ones keep the vector form).  gfa_autotune times both with
- * its fastest tile shape and keeps the faster.  No counterpart in the reference. */
-int gfa_set_projection_engine(gfa_plan* plan, int fp_bits, int engine);
-int gfa_get_projection_engine(gfa_plan* plan, int fp_bits, int* engine);
+/* the engine of the dyadic term's two contractions per precision, the SHO projection (c = P^T x over the blocks an
+ * atom image touches) and the add-back (y += P d): 0 their sum-factorised forms on the vector ALUs, the add-back
+ * inside the local kernel (the default); 1 their dense forms on the matrix cores (16x16x4 MFMA of the precision,
+ * FP32 or FP64), the add-back in a kernel of its own after the local kernel.  Atoms with numax > 5 keep the vector
+ * forms.  gfa_autotune times both with its fastest tile shape and keeps the faster.  No counterpart in the
+ * reference. */
+int gfa_set_dyadic_engine(gfa_plan* plan, int fp_bits, int engine);
+int gfa_get_dyadic_engine(gfa_plan* plan, int fp_bits, int* engine);
 /* the tile shape the action of fp_bits uses (the autotuner's choice, the pinned one or the default); *tuned = 1
  * when it was set by gfa_autotune or gfa_set_tile_shape */
 int gfa_get_tile_shape(gfa_plan* plan, int fp_bits, int* kind, int* columns, int* lines, int* tuned);
@@ -201,6 +203,10 @@ typedef struct {
                                  * cell; one application per iteration, no squared residual polynomial, so its FP32
                                  * recurrences keep converging where tfQMR's stall)                                 */
     int32_t reserved;
+    const int32_t* group_max_iterations; /* optional [nCols]: a column group leaves the solve after this many
+                                 * operator applications (0: no cap); with a tolerance far below reach every group
+                                 * then stops at a fixed iterate (solves of nearby operators stop at the same
+                                 * one: finite differences), while the others run on                         */
 } gfa_solve_opts;
 
 typedef struct {
@@ -212,6 +218,8 @@ typedef struct {
     double seconds;              /* wall time of the solve                                           */
     double flops;                /* action flops (reference counting)                                */
     int32_t actions;             /* number of action applications                                    */
+    int32_t host_syncs;          /* host waits on the device (status polls, residual fetches, set-ups)     */
+    int32_t graph_captures;      /* HIP graph captures of a solver step (one per change of the active set) */
     int32_t reserved;
 } gfa_solve_stats;
 

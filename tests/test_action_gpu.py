@@ -116,10 +116,11 @@ def test_every_tile_shape_matches_oracle(device, kind, columns, lines):
 
 
 @pytest.mark.parametrize("kind,columns,lines", [(1, 32, 0), (2, 32, 2), (0, 16, 2)])
-def test_matrix_core_projection_matches_oracle(device, kind, columns, lines):
-    """The dense SHO projection on the matrix cores (16x16x4 MFMA in the action's precision) gives the oracle's
-    action (FP64 1e-12, FP32 2e-5) with periodic images at a k-point, the 16th-order stencil, mixed boundary
-    conditions and numax 5 atoms."""
+def test_matrix_core_dyadic_matches_oracle(device, kind, columns, lines):
+    """The dense SHO projection and add-back on the matrix cores (16x16x4 MFMA in the action's precision, the
+    add-back in its own kernel after the local one) give the oracle's action (FP64 1e-12, FP32 2e-5) with periodic
+    images at a k-point, the 16th-order stencil, mixed boundary conditions and numax 5 atoms; and a mixed COCG
+    solve with them (its (p, A p) then comes from a pass of its own) meets the residual 1e-9."""
     import torch
     geoms = [c for c in cases() if c[0] in ("periodic8_k", "isolated6_fd16", "mixed_bc", "wrap16_numax5")]
     for name, mk, kw, E, k in geoms:
@@ -129,15 +130,21 @@ def test_matrix_core_projection_matches_oracle(device, kind, columns, lines):
         y_ref = O.Oracle(plan, E, k).action(x)
         for fp, tol, dt in ((64, 1e-12, torch.float64), (32, 2e-5, torch.float32)):
             plan.set_tile_shape(fp, kind, columns, lines)
-            plan.set_projection_engine(fp, 1)
-            assert plan.projection_engine(fp) == 1
+            plan.set_dyadic_engine(fp, 1)
+            assert plan.dyadic_engine(fp) == 1
             xd = torch.from_numpy(x).to(device=device, dtype=dt)
             yd = torch.empty_like(xd)
-            print(f"mfma projection, tile {kind}:{columns}x{lines} {name} fp{fp}", flush=True)
+            print(f"matrix-core dyadic term, tile {kind}:{columns}x{lines} {name} fp{fp}", flush=True)
             plan.multiply(yd, xd, fp_bits=fp)
             torch.cuda.synchronize()
             err = np.abs(yd.double().cpu().numpy() - y_ref).max() / np.abs(y_ref).max()
             assert err < tol, f"{name} fp{fp}: {err:.3e}"
+        if name == "wrap16_numax5":
+            _, st = plan.solve(fp_bits=32, tolerance=1e-9, max_iterations=3000, max_refinements=6,
+                               inner_tolerance=1e-30, method="cocg")
+            print(f"  mixed COCG with the matrix-core dyadic term: {st['converged_columns']}/{st['total_columns']}, "
+                  f"max residual {st['max_residual']:.2e}", flush=True)
+            assert st["converged_columns"] == st["total_columns"] and st["max_residual"] <= 1e-9
         plan.close()
 
 

@@ -204,58 +204,162 @@ def _unit(V, p):
 
 
 @pytest.mark.gpu
+@pytest.mark.timeout(900)
 def test_autograd_on_the_16_cube_contour(device):
-    """The adjoint gradient on the 16^3-block cell (64^3 grid, 16 atoms numax 3), 8 contour points on one source
-    block (radius 6 Bohr, wrapped), against central differences of GPU forward solves (COCG to 1e-14) at 32 seeded
-    grid points around the source and 8 entries of the contributing atoms' matrices (the four largest gradient
-    components and four seeded ones).
+    """The adjoint gradient on the 16^3-block cell (64^3 grid, 16 atoms numax 3) over 8 contour points on the half
+    ellipse of the batched contour workload (E from -1 to 0 Ha, Im E from 0.005 to 0.3 Ha), one source block,
+    radius 6 Bohr (wrapped).
 
-    Two steps per direction.  At step 1e-6 Ha the quotient carries the roundoff noise of the FP64 forward solves
-    of L (~5e-17 absolute here, L ~ 1e-2; tighter solve tolerances do not lower it) divided by the step: ~6e-11,
-    which is 1e-6 of the largest gradient component; the bound there is 2e-6 of it.  L is smooth in V and in the
-    matrices, so at step 1e-4 Ha (8 of the grid points, all matrix entries) the noise is 100 times smaller (~1e-8 of the largest component) and the
-    truncation error (step^2) still below it: there the bound is 3e-8 of the largest component."""
+    Every solve runs a fixed number of COCG iterations per contour point (those the unperturbed forward solve needs
+    for a residual of 1e-14, plus 64) and no stopping test, so the forward solves at V + step and V - step stop at
+    the same iterates and their difference carries no stopping-point noise.
+    (1) Central differences at step 1e-6 Ha at 32 seeded grid points around the source and 8 matrix entries (the
+    four largest components and four seeded ones): the difference quotient, evaluated exactly from the two
+    perturbed solutions by the resolvent identity, equals the adjoint gradient to 1e-6 of its largest component;
+    the plain quotient (L+ - L-) / (2 step) is reported and bounded by 1e-4 only: it subtracts two values of L that
+    agree to 14 digits, so its FP64 rounding is up to 3e-5 of the largest component here.
+    (2) The CPU FP64 adjoint from the device X and lambda: dL/dV = -Im sum_c w_c sum_j conj(lambda) G folded onto
+    the grid (numpy), and dL/dh, dL/ds of the four largest matrix entries from oracle actions of the dyadic term
+    (exact differences: A is affine in h, and s enters as -E_c times the same operator), to 1e-10."""
     import torch
     from angstromcube_amd import adjoint as A
     sysm = S.contour_16()
     x, wl = np.polynomial.legendre.leggauss(8)
     phi = 0.5 * np.pi * (x + 1.0)
-    E = -0.5 - 0.5 * np.cos(phi) + 1j * (0.05 + 0.25 * np.sin(phi))
-    w = wl * (0.5 * np.sin(phi) + 1j * 0.25 * np.cos(phi)) * 0.5 * np.pi
+    E = -0.5 - 0.5 * np.cos(phi) + 1j * (0.005 + 0.295 * np.sin(phi))
+    w = wl * (0.5 * np.sin(phi) + 1j * 0.295 * np.cos(phi)) * 0.5 * np.pi
     plan = G.GreenPlan(sysm, r_trunc=6.0, sources=[(8, 8, 8)] * 8)
     assert plan.nCols == 8
+    plan.update_potential()
+    plan.update_energies(E)
+    _, st0 = plan.solve(fp_bits=64, tolerance=1e-14, max_iterations=20000, max_refinements=0, method="cocg")
+    caps = st0["column_halfsteps"].reshape(8, 64).max(axis=1) + 64
+    print(f"16^3 contour: nnzb {plan.nnzb}, iterations per contour point {caps.tolist()}", flush=True)
     V = torch.from_numpy(sysm.V.copy()).requires_grad_(True)
     hs = torch.from_numpy(A.flatten_hs(sysm.hs)).requires_grad_(True)
-    L = A.contour_trace(V, hs, plan, E, w, tol=1e-14, method="cocg")
+    kw = dict(tol=1e-13, method="cocg", fixed=caps)
+    L = A.contour_trace(V, hs, plan, E, w, **kw)
     gV, ghs = torch.autograd.grad(L, (V, hs))
-    print(f"16^3 contour: nnzb {plan.nnzb}, L = {float(L.detach()):.10e}, max |dL/dV| {float(gV.abs().max()):.3e}, "
-          f"max |dL/dhs| {float(ghs.abs().max()):.3e}", flush=True)
+    gVmax, ghmax = float(gV.abs().max()), float(ghs.abs().max())
+    print(f"  L = {float(L.detach()):.12e}, max |dL/dV| {gVmax:.3e}, max |dL/dhs| {ghmax:.3e}", flush=True)
+
+    # (2) the CPU FP64 adjoint from the device solutions
+    Xd, _ = A.solve_fp64(plan, E, 1e-13, 0, "cocg", caps)
+    Ld, _ = A.solve_fp64(plan, np.conj(E), 1e-13, 0, "cocg", caps)
+    Xh, Lh = Xd.cpu().numpy(), Ld.cpu().numpy()
+    del Xd, Ld
+    cols, veff = plan.export("COLINDX").astype(int), plan.export("VEFF_INDEX")
+    s = (Lh[:, 0] * Xh[:, 0] + Lh[:, 1] * Xh[:, 1]).sum(axis=2) + 1j * (Lh[:, 0] * Xh[:, 1] - Lh[:, 1] * Xh[:, 0]).sum(axis=2)
+    g = np.zeros((int(np.prod([n // 4 for n in sysm.ng])), 64))
+    inside = veff >= 0
+    np.add.at(g, veff[inside], -np.imag(w[cols[inside], None] * s[inside]))
+    gV_cpu = cell_to_grid(g, [n // 4 for n in sysm.ng])
+    err = np.abs(gV.numpy() - gV_cpu).max() / np.abs(gV_cpu).max()
+    print(f"  dL/dV: GPU vs CPU FP64 contraction {err:.2e}", flush=True)
+    assert err <= 1e-10
+    offs = np.cumsum([0] + [2 * m.shape[-1] ** 2 for m in sysm.hs])
+    top = [int(e) for e in np.argsort(-ghs.abs().numpy())[:4]]
+    base = O.Oracle(plan, 0j).action(Xh, which=4)
+    Xc = Xh[:, 0] + 1j * Xh[:, 1]
+    Lc = Lh[:, 0] + 1j * Lh[:, 1]
+    for e in top:
+        ia = int(np.searchsorted(offs, e, side="right") - 1)
+        nc = sysm.hs[ia].shape[-1]
+        which, kl = divmod(e - int(offs[ia]), nc * nc)
+        k, l = divmod(kl, nc)
+        s_plus = copy.deepcopy(sysm)
+        s_plus.hs[ia] = s_plus.hs[ia].copy()
+        s_plus.hs[ia][0, k, l] += 1.0   # the h entry: the dyadic operator of e_kl
+        d = O.Oracle(_PlanView(plan, s_plus), 0j).action(Xh, which=4) - base
+        dAX = d[:, 0] + 1j * d[:, 1]
+        if which == 1:
+            dAX = dAX * (-E[cols])[:, None, None]   # s enters the operator as -E_c s
+        z = (np.conj(Lc) * dAX).sum(axis=(1, 2))
+        ref = float(np.sum(-np.imag(w[cols] * z)))
+        got = float(ghs[e])
+        print(f"  dL/d{'hs'[which]}[{ia}][{k},{l}]: GPU {got:.12e} CPU {ref:.12e}", flush=True)
+        assert abs(got - ref) <= 1e-10 * ghmax
+
+    # (1) central differences at step 1e-6 Ha.  The plain quotient (L+ - L-) / (2 step) subtracts two values of L
+    # that agree to about 14 digits, so it carries the FP64 rounding of the two solves (~1e-16 in L+ - L-, up to
+    # 3e-5 of the largest component here).  The same quotient is also evaluated without that cancellation, from
+    # the two perturbed solutions, by the resolvent identity G+ - G- = -G+ (A+ - A-) G- = -2 step G+ D G-:
+    #   (L+ - L-) / (2 step) = -Im sum_c w_c Tr_src[G+_c D G-_c] = -Im sum_c w_c sum_j X+_c[:, j]^T D X-_c[:, j]
+    # (G^T = G: the operator is complex symmetric, Gamma and a wrapped cell).  For V at one grid point, D is the
+    # unit diagonal at that point of every block reading its cell; for a matrix entry, D = dA/dh_kl comes from
+    # oracle actions of the dyadic term (and dA/ds_kl = -E_c dA/dh_kl); for an off-diagonal entry A+ is not
+    # symmetric and X+ comes from the solve with the transposed entry perturbed.  This exact quotient is held to
+    # 1e-6 of the largest component; the plain one only to 1e-4 (its rounding reaches 3e-5).
     rng = np.random.default_rng(16)
     centre = 8 * 4 + 2   # grid index of the source block's centre along each axis
     pts = [tuple(int(centre + rng.integers(-10, 11)) for _ in range(3)) for _ in range(32)]
     ids = set(plan.export("ATOM_IDS").astype(int).tolist())
-    offs = np.cumsum([0] + [2 * m.shape[-1] ** 2 for m in sysm.hs])
-    ents = [int(e) for e in np.argsort(-ghs.abs().numpy())[:4]]
+    ents = list(top)
     while len(ents) < 8:
         ia = int(rng.integers(sysm.natoms))
         if ia in ids:
             ents.append(int(offs[ia] + rng.integers(offs[ia + 1] - offs[ia])))
-    # step 1e-4 on the first 8 grid points and on every matrix entry (keeps the test near a minute)
-    cases = [("V", p, gV, lambda d: (V + d, hs), i < 8) for i, p in enumerate(pts)] + \
-        [("hs", e, ghs, lambda d: (V, hs + d), True) for e in ents]
-    worst = {1e-6: 0.0, 1e-4: 0.0}
+    nbx, nby = sysm.ng[2] // 4, sysm.ng[1] // 4
+    veff_t = torch.from_numpy(veff.astype(np.int64)).to(device)
+    cols_t = torch.from_numpy(cols).to(device)
+    w_t = torch.from_numpy(w).to(device)
+    Ec_t = torch.from_numpy(E).to(device)
+
+    def solve_at(Vt, hst):
+        plan.update_potential(Vt.numpy(), A._split_hs(sysm, hst.numpy()))
+        X, rho = A.solve_fp64(plan, E, 1e-13, 0, "cocg", caps)
+        return A.trace_value(rho, w), X
+
+    def cplx(X):
+        return torch.complex(X[:, 0], X[:, 1])
+
+    step, worst = 1e-6, {"plain": 0.0, "exact": 0.0}
     with torch.no_grad():
-        for name, idx, g, args, both in cases:
-            u = torch.zeros_like(g)
+        for name, idx, gg in [("V", p, gV) for p in pts] + [("hs", e, ghs) for e in ents]:
+            u = torch.zeros_like(gg)
             u[idx] = 1.0
-            fds = {1e-4: float("nan")}
-            for step in ((1e-6, 1e-4) if both else (1e-6,)):
-                Lp, Lm = [float(A.contour_trace(*args(sgn * step * u), plan, E, w, tol=1e-14, method="cocg"))
-                          for sgn in (+1, -1)]
-                fds[step] = (Lp - Lm) / (2 * step)
-                worst[step] = max(worst[step], abs(fds[step] - float(g[idx])) / float(g.abs().max()))
-            print(f"{name}[{idx}]: adjoint {float(g[idx]):.10e} fd(1e-6) {fds[1e-6]:.10e} fd(1e-4) {fds[1e-4]:.10e}",
-                  flush=True)
-            assert abs(fds[1e-6] - float(g[idx])) <= 2e-6 * float(g.abs().max())
-            assert not both or abs(fds[1e-4] - float(g[idx])) <= 3e-8 * float(g.abs().max())
-    print(f"largest deviation / largest component: step 1e-6 {worst[1e-6]:.2e}, step 1e-4 {worst[1e-4]:.2e}")
+            if name == "V":
+                Lp, Xp = solve_at(V + step * u, hs)
+                Lm, Xm = solve_at(V - step * u, hs)
+                z, y, xg = idx
+                cell = ((z // 4) * nby + y // 4) * nbx + xg // 4
+                i = (z % 4) * 16 + (y % 4) * 4 + xg % 4
+                sel = torch.nonzero(veff_t == cell).flatten()
+                prod = (cplx(Xp[sel])[:, i, :] * cplx(Xm[sel])[:, i, :]).sum(dim=1)   # per block, sum over j
+                exact = float(-torch.imag((w_t[cols_t[sel]] * prod).sum()))
+            else:
+                Lp, Xp = solve_at(V, hs + step * u)
+                Lm, Xm = solve_at(V, hs - step * u)
+                ia = int(np.searchsorted(offs, idx, side="right") - 1)
+                nc = sysm.hs[ia].shape[-1]
+                which, kl = divmod(idx - int(offs[ia]), nc * nc)
+                k, l = divmod(kl, nc)
+                if k != l:
+                    # one off-diagonal entry makes A+ unsymmetric: row s of G+ is column s of (A+)^-T, the solve with
+                    # the transposed entry perturbed
+                    uT = torch.zeros_like(gg)
+                    uT[int(offs[ia]) + which * nc * nc + l * nc + k] = 1.0
+                    del Xp
+                    _, Xp = solve_at(V, hs + step * uT)
+                Xmh = Xm.cpu().numpy()
+                s_plus = copy.deepcopy(sysm)
+                s_plus.hs[ia] = s_plus.hs[ia].copy()
+                s_plus.hs[ia][0, k, l] += 1.0
+                d = O.Oracle(_PlanView(plan, s_plus), 0j).action(Xmh, which=4) - O.Oracle(plan, 0j).action(Xmh, which=4)
+                DXm = torch.from_numpy(d).to(device)
+                scale = w_t[cols_t] * ((-Ec_t[cols_t]) if which == 1 else 1.0)
+                prod = (cplx(Xp) * cplx(DXm)).sum(dim=(1, 2))
+                exact = float(-torch.imag((scale * prod).sum()))
+                del DXm, d, Xmh
+            plain = (Lp - Lm) / (2 * step)
+            g0, gmax = float(gg[idx]), float(gg.abs().max())
+            dev_p, dev_e = abs(plain - g0) / gmax, abs(exact - g0) / gmax
+            worst["plain"] = max(worst["plain"], dev_p)
+            worst["exact"] = max(worst["exact"], dev_e)
+            print(f"  {name}[{idx}]: adjoint {g0:.10e} quotient {plain:.10e} ({dev_p:.1e}) exact quotient "
+                  f"{exact:.10e} ({dev_e:.1e})", flush=True)
+            assert dev_e <= 1e-6
+            assert dev_p <= 1e-4
+            del Xp, Xm
+    print(f"  largest deviation / largest component: plain quotient {worst['plain']:.2e}, exact quotient "
+          f"{worst['exact']:.2e}")

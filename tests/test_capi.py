@@ -47,11 +47,11 @@ def test_default_and_pinned_tile_shapes_without_gpu():
     assert p.tile_shape(64) == (0, 16, 2, 0) and p.tile_shape(32) == (1, 32, 0, 0)
     p.set_tile_shape(64, 1, 64)
     assert p.tile_shape(64) == (1, 64, 0, 1)
-    assert p.projection_engine(32) == 0 and p.projection_engine(64) == 0
-    p.set_projection_engine(32, 1)
-    assert p.projection_engine(32) == 1
+    assert p.dyadic_engine(32) == 0 and p.dyadic_engine(64) == 0
+    p.set_dyadic_engine(32, 1)
+    assert p.dyadic_engine(32) == 1
     with pytest.raises(G.GfaError, match="engine"):
-        p.set_projection_engine(64, 2)
+        p.set_dyadic_engine(64, 2)
     p.set_tile_shape(32, 2, 64, 2)
     assert p.tile_shape(32) == (2, 64, 2, 1)
     with pytest.raises(G.GfaError, match="tile"):
