@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 
@@ -1946,6 +1947,91 @@ template int device_action<double>(DevicePlan&, ActionScalars const&, double*, d
 template int device_action<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t, std::string&,
                                   int, double*, Workspace<float>*, double*);
 
+// ---------------------------------------------------------------------------------------------
+// The action without allocation (gfa_action_noalloc): the projection partial sums, the addition coefficients,
+// the atom-multiply scratch of numax 6, 7 atoms and the zero block live in `scratch`, a fixed array of every GPU
+// the library runs on (held until the process ends).  Its device address is taken once per device with
+// hipGetSymbolAddress; the host-side name `scratch` is never handed to a kernel.  Calls that overlap on one GPU
+// share the array and race.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// the address of `scratch` on the current device (which must be `device`), looked up on its first use there
+int scratch_address(int device, unsigned char** out, std::string& err) {
+    if (device < 0 || device >= MAX_DEVICES) { err = "device ordinal out of range"; return 1; }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t const e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return 2;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[device]);
+    return 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_zero(T* __restrict__ p, uint32_t n) {
+    uint32_t const i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+size_t noalloc_scratch_bytes(DevicePlan const& d, int fp_bits) {
+    size_t const s = fp_bits == 64 ? 8 : 4;
+    size_t b = round256(size_t(NBLK) * s);   // zero block
+    if (d.nAtomImages > 0) {
+        b += round256(size_t(d.total_chunk_coefs) * 128 * s);
+        b += round256(size_t(d.natomcoeffs) * d.nCols * 128 * s);
+        if (d.nchunks > 0 && d.max_atom_nc > 64) b += round256(size_t(d.ncoeffs) * d.nCols * 128 * 8);
+    }
+    return b;
+}
+
+template <typename real_t>
+int device_action_noalloc(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
+                          std::string& err) {
+    size_t const need = noalloc_scratch_bytes(d, sizeof(real_t) == 8 ? 64 : 32);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        err = "the plan needs " + std::to_string(need) + " bytes of action scratch, more than the " +
+              std::to_string(NOALLOC_SCRATCH_BYTES) + " of the allocation-free action";
+        return 1;
+    }
+    unsigned char* base = nullptr;
+    if (int r = scratch_address(d.device, &base, err)) return r;
+    unsigned char* cur = base;
+    auto take = [&](size_t bytes) { unsigned char* p = cur; cur += round256(bytes); return p; };
+    Workspace<real_t> ws;
+    ws.zero = reinterpret_cast<real_t*>(take(size_t(NBLK) * sizeof(real_t)));
+    double* csc = nullptr;
+    if (d.nAtomImages > 0) {
+        ws.partial = reinterpret_cast<real_t*>(take(size_t(d.total_chunk_coefs) * 128 * sizeof(real_t)));
+        ws.dcoef = reinterpret_cast<real_t*>(take(size_t(d.natomcoeffs) * d.nCols * 128 * sizeof(real_t)));
+        if (d.nchunks > 0 && d.max_atom_nc > 64)
+            csc = reinterpret_cast<double*>(take(size_t(d.ncoeffs) * d.nCols * 128 * 8));
+    }
+    // the array keeps the previous call's contents: the zero block is cleared on every call (a kernel, so that
+    // the call can be captured in a graph); the other regions are written before they are read
+    hipLaunchKernelGGL((k_fill_zero<real_t>), dim3((NBLK + 255) / 256), dim3(256), 0, stream, ws.zero, uint32_t(NBLK));
+    HIP_TRY(hipGetLastError());
+    return device_action<real_t>(d, s, y, x, stream, err, ACTION_ALL, nullptr, &ws, csc);
+}
+template int device_action_noalloc<double>(DevicePlan&, ActionScalars const&, double*, double const*, hipStream_t,
+                                           std::string&);
+template int device_action_noalloc<float>(DevicePlan&, ActionScalars const&, float*, float const*, hipStream_t,
+                                          std::string&);
 
 // Times every k_local tile shape on the caller's vectors (the whole action, reps launches each, HIP events on
 // `stream`) and keeps the fastest for this precision.  y is overwritten.

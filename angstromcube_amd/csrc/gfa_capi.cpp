@@ -296,6 +296,27 @@ int gfa_action(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream
     });
 }
 
+int gfa_action_noalloc(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, double* nflops) {
+    return guarded([&]() -> int {
+        if (!plan || !y || !x) return fail(GFA_ERR_ARGUMENT, "null argument");
+        if (fp_bits != 64 && fp_bits != 32) return fail(GFA_ERR_ARGUMENT, "fp_bits must be 32 or 64");
+        if (y == x) return fail(GFA_ERR_ARGUMENT, "the action cannot run in place");
+        if (!plan->energy_set) return fail(GFA_ERR_STATE, "set the energy before applying the action");
+        if (!plan->dev.uploaded)
+            return fail(GFA_ERR_STATE, "the allocation-free action needs the plan on the device: call gfa_plan_prepare "
+                                       "(or any other device entry point) first");
+        if (int s = ensure_device(plan)) return s;   // uploaded: copies changed values only, allocates nothing
+        std::string err;
+        auto const sc = scalars(plan->host);
+        int const r = fp_bits == 64
+            ? gfa::device_action_noalloc<double>(plan->dev, sc, (double*)y, (double const*)x, (hipStream_t)stream, err)
+            : gfa::device_action_noalloc<float>(plan->dev, sc, (float*)y, (float const*)x, (hipStream_t)stream, err);
+        if (r) return fail(r == 1 ? GFA_ERR_ARGUMENT : GFA_ERR_DEVICE, err);
+        if (nflops) *nflops = gfa::reference_flops(plan->host, fp_bits);
+        return GFA_OK;
+    });
+}
+
 int gfa_action_part(gfa_plan* plan, int fp_bits, int part, void* y, const void* x, void* stream) {
     return guarded([&]() -> int {
         if (!plan || !y || !x) return fail(GFA_ERR_ARGUMENT, "null argument");

@@ -139,6 +139,14 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
                   std::string& err, int parts = ACTION_ALL, double* dot = nullptr, Workspace<real_t>* ext = nullptr,
                   double* ext_cscratch = nullptr);
 
+// the whole action with its workspace in the library's fixed per-device scratch array (no allocation); the plan
+// must be on the device already.  Returns 1 if the plan needs more than NOALLOC_SCRATCH_BYTES.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+size_t noalloc_scratch_bytes(DevicePlan const& d, int fp_bits);
+template <typename real_t>
+int device_action_noalloc(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream,
+                          std::string& err);
+
 // flop count of one action as the reference counts it (include/green_action.hxx:154-175)
 template <typename real_t>
 int device_autotune(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const* x, hipStream_t stream, int reps,

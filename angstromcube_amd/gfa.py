@@ -24,7 +24,7 @@ EXPORTS = ["ROW_START", "COLINDX", "SUBSET", "VEFF_INDEX", "ROW_POS", "COL_POS",
 EXPORT_ID = {n: i for i, n in enumerate(EXPORTS)}
 
 C_SYMBOLS = ["gfa_plan_create", "gfa_plan_destroy", "gfa_plan_query", "gfa_plan_export", "gfa_set_potential",
-             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_action_benchmark", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_set_dyadic_engine", "gfa_get_dyadic_engine", "gfa_action_part", "gfa_gradient",
+             "gfa_set_atom_matrices", "gfa_set_energy", "gfa_set_energies", "gfa_set_kpoint", "gfa_action", "gfa_action_multiply", "gfa_action_benchmark", "gfa_action_noalloc", "gfa_autotune", "gfa_set_tile_shape", "gfa_get_tile_shape", "gfa_set_dyadic_engine", "gfa_get_dyadic_engine", "gfa_action_part", "gfa_gradient",
              "gfa_solve", "gfa_solver_release", "gfa_solve_memory", "gfa_plan_prepare", "gfa_atom_density_matrices", "gfa_gauss_legendre", "gfa_gauss_fermi_dirac", "gfa_energy_mesh", "gfa_contour_density", "gfa_solve_chunked",
              "gfa_last_error"]
 
@@ -117,6 +117,7 @@ def lib():
         L.gfa_action_multiply.argtypes = [vp, C.c_int, vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int,
                                           C.POINTER(C.c_double)]
         L.gfa_action_benchmark.argtypes = [vp, C.c_int, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gfa_action_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_double)]
         L.gfa_autotune.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(C.c_double)]
         L.gfa_set_tile_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         ip = C.POINTER(C.c_int)

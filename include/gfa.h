@@ -142,6 +142,15 @@ int gfa_action_multiply(gfa_plan* plan, int fp_bits, void* y, const void* x, con
                         uint32_t nnzb, uint32_t nCols, uint32_t l2nX, void* stream, int precondition,
                         double* nflops);
 
+/* ---- the action without device allocation: no counterpart in the reference.  For callers that cannot
+ *      allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ *      kernels and results as gfa_action, with the action's workspace (projection partial sums, atom
+ *      coefficients, the zero block) in a fixed scratch array of 64 MiB that the library holds on every GPU it
+ *      runs on until the process ends.  The plan must be on the device already (gfa_plan_prepare, or any earlier
+ *      device call); a plan whose workspace exceeds 64 MiB is refused with GFA_ERR_ARGUMENT.  Calls that overlap
+ *      on one GPU share the array and race: serialise them (one stream, or events). ---- */
+int gfa_action_noalloc(gfa_plan* plan, int fp_bits, void* y, const void* x, void* stream, double* nflops);
+
 /* ---- the reference's action timing loop: action_t::solve with a negative iteration count and no tfQMRgpu
  *      (include/green_action.hxx:243-281, called by test_action at src/green_action.cxx:139-156).  Starting from
  *      x = 0 it applies y = A x `iterations` times, swapping x and y, with a device synchronisation after every
@@ -234,7 +243,7 @@ int gfa_solve(gfa_plan* plan, const gfa_solve_opts* opts, double* rho, gfa_solve
  *      largest -- for fp_bits 32 including the FP64 work vectors of a column group that falls back to FP64 -- and
  *      the bytes of the plan's device data (an estimate before the plan is first used on the device).
  *      gfa_plan_prepare uploads the plan and allocates the action's workspace of fp_bits, so that later actions
- *      of that precision allocate nothing (for timing and graph capture). ---- */
+ *      of that precision allocate nothing (for timing, graph capture and gfa_action_noalloc). ---- */
 int gfa_solver_release(gfa_plan* plan);
 int gfa_solve_memory(gfa_plan* plan, const gfa_solve_opts* opts, uint64_t* solver_bytes, uint64_t* plan_bytes);
 int gfa_plan_prepare(gfa_plan* plan, int fp_bits);
