@@ -62,9 +62,10 @@ def parse():
     ap.add_argument("--cpu-halfsteps", type=int, default=10, help="half-steps of the timed CPU TFQMR sample")
     ap.add_argument("--budget-s", type=float, default=420.0,
                     help="wall seconds after which the optional batch-0 comparisons are skipped")
-    ap.add_argument("--mode", choices=["bench", "action", "contour", "batch128", "batch"], default="bench",
-                    help="action: only apply the action a few times (for profiler runs); batch128: the 128 "
-                         "instances in one plan; batch: one mixed solve of the batch --batch (for profiler runs)")
+    ap.add_argument("--mode", choices=["bench", "action", "dyadic", "contour", "batch128", "batch"], default="bench",
+                    help="action: only apply the action a few times (for profiler runs); dyadic: the action with "
+                         "each engine of the dyadic term (for counter runs); batch128: the 128 instances in one plan; "
+                         "batch: one mixed solve of the batch --batch (for profiler runs)")
     ap.add_argument("--batch", type=int, default=0, help="k-point batch of --mode batch")
     return ap.parse_args()
 
@@ -136,6 +137,10 @@ def main():
 
     if args.mode == "action":
         action_mode(G, sysm, instances, n, args, stream, dev)
+        return
+    if args.mode == "dyadic":
+        p = batch_plan(G, sysm, batch(instances, 0, n))
+        dyadic_engines(p, args, stream)
         return
     if args.mode == "contour":
         contour_mode(G, sysm, args)
@@ -351,13 +356,7 @@ def action_mode(G, sysm, instances, n, args, stream, dev):
         p.set_tile_shape(64, *shape)
         t = time_action(p, 64, 2, args.action_reps, stream)
         log(f"# local fp64 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
-    for fp in (32, 64):   # the dyadic term's engines: part 1 (projection and atom matrices), part 2 (local kernel
-        for eng in (0, 1):  # with the add-back inside it, or followed by the matrix-core add-back) and the whole action
-            p.set_dyadic_engine(fp, eng)
-            t1, t2, t3 = (time_action(p, fp, part, args.action_reps, stream) for part in (1, 2, 3))
-            log(f"# dyadic engine {('vector', 'matrix')[eng]} fp{fp}: projection {t1 * 1e3:.3f} ms, local + add-back "
-                f"{t2 * 1e3:.3f} ms, action {t3 * 1e3:.3f} ms")
-        p.set_dyadic_engine(fp, 0)
+    dyadic_engines(p, args, stream)
     # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
     x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
     y = torch.empty_like(x)
@@ -365,6 +364,18 @@ def action_mode(G, sysm, instances, n, args, stream, dev):
         y.copy_(x)
     torch.cuda.synchronize()
     log(f"# copy calibration: {x.numel() * 8} bytes read and written per copy")
+
+
+def dyadic_engines(p, args, stream):
+    """The dyadic term's engines on plan p: part 1 (projection and atom matrices), part 2 (local kernel with the
+    add-back inside it, or followed by the matrix-core add-back) and the whole action, in FP32 and FP64."""
+    for fp in (32, 64):
+        for eng in (0, 1):
+            p.set_dyadic_engine(fp, eng)
+            t1, t2, t3 = (time_action(p, fp, part, args.action_reps, stream) for part in (1, 2, 3))
+            log(f"# dyadic engine {('vector', 'matrix')[eng]} fp{fp}: projection {t1 * 1e3:.3f} ms, local + add-back "
+                f"{t2 * 1e3:.3f} ms, action {t3 * 1e3:.3f} ms")
+        p.set_dyadic_engine(fp, 0)
 
 
 def batch_mode(G, sysm, instances, n, args, opts):
