@@ -379,13 +379,28 @@ __global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
 //    branch-free addressing (an absent neighbour reads a zero block) so that all loads of a line are in flight
 //    together.
 // ---------------------------------------------------------------------------------------------
+// word indices in a block header: block, column group, potential cell (-1: none), add-list range [ADD0, ADD1),
+// then the neighbour codes [direction][side][k < NH] (>= 0 block, -1 absent, <= -2 periodic image -code - 2)
+enum : int { IH_BLK = 0, IH_COL = 1, IH_VEFF = 2, IH_ADD0 = 3, IH_ADD1 = 4, IH_NBR = 5 };
+template <int NH>
+struct alignas(64) ItemHead {
+    uint32_t w[NH == 1 ? 16 : 32];
+    __device__ int32_t code(int d, int sd, int k) const { return int32_t(w[IH_NBR + (d * 2 + sd) * NH + k]); }
+};
+
+// one add-list entry: atom image, its numax, the Hermite slot of (image, target row), its first coefficient
+struct AddEntry {
+    uint32_t img, L, slot, coef0;
+};
+
 template <typename real_t>
 struct LocalArgs {
     real_t* y;
     real_t const* x;
     real_t const* dcoef;
     real_t const* zero;      // one block of zeros
-    uint32_t const* order;   // processing order of the blocks (cache-aware)
+    uint32_t const* items;   // block headers in processing order (cache-aware), ItemHead<NH> each
+    AddEntry const* add_entry;
     int32_t const* veff;
     int32_t const* nbr;      // [3][2][NH][nnzb]
     double const* V;
@@ -546,7 +561,8 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
     uint64_t const item = lin % a.nitems;
     int const slice = int(lin / a.nitems);
     if (slice >= NSLICE) return;  // uniform over the work-group
-    uint64_t const blk = a.order[item];
+    ItemHead<NH> const h = reinterpret_cast<ItemHead<NH> const*>(a.items)[item];   // one scalar load
+    uint64_t const blk = h.w[IH_BLK];
     int const tid = threadIdx.x;
 
     // stage this column slice of the block in LDS with 16-byte loads
@@ -564,10 +580,10 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
 
     // ---- potential: ys = (V - E) x ----
     {
-        int32_t const iv = a.veff[blk];
+        int32_t const iv = int32_t(h.w[IH_VEFF]);
         real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
         double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64;
-        uint32_t const colE = a.colindx[blk];
+        uint32_t const colE = h.w[IH_COL];
         real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
 #pragma unroll
         for (int k = 0; k < 2 * 64 * JW / T; ++k) {
@@ -595,7 +611,7 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
         for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
             for (int k = 0; k < NH; ++k) {
-                int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+                int32_t const code = h.code(d, sd, k);
                 uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
                 nb[sd][k] = (code == -1) ? a.zero : a.x + idx * NBLK;
                 bool const img = IMG && (code <= -2);
@@ -655,21 +671,21 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
 
     // ---- SHO add-back of the atom images (numax <= 5) touching this block; the coefficients of
     //      each image are staged in the LDS buffer of the block, which is free after the kinetic passes ----
-    uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
+    uint32_t const b0 = h.w[IH_ADD0], b1 = h.w[IH_ADD1];
     if (a.add_local && b1 > b0) {
-        uint32_t const col = a.colindx[blk];
+        uint32_t const col = h.w[IH_COL];
         float const R2 = float(a.hgrid[3] * a.hgrid[3]);
         int const zq = lg;  // threads (jl, p, zq) own the z plane zq
         real_t* cs = xs;    // [nc][2][JW]
         for (uint32_t b = b0; b < b1; ++b) {
-            uint32_t const img = a.add_index[b];
-            int const L = a.image_lmax[img];
+            AddEntry const e = a.add_entry[b];
+            int const L = int(e.L);
             if (L > 5) continue;  // done by k_sho_add_wide
             int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
             __syncthreads();
-            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
+            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, e.slot, L, tid, T);
             {
-                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
+                real_t const* src = a.dcoef + (size_t(e.coef0) * a.nCols + col) * 128 + slice * JW;
                 for (int v = tid; v < 2 * nc * VPR; v += T) {
                     int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
                     *reinterpret_cast<vec_t*>(cs + row * JW + cv * VE) =
@@ -774,13 +790,13 @@ __device__ __forceinline__ void z_pass(LocalArgs<real_t> const& a, real_t (&acc)
 // neighbour blocks of blk in direction d, NH per side: base pointers (an absent neighbour reads the zero block)
 // and the Bloch phase of a periodic image as (fr, fi) for plane p (fr = 1, fi = 0 otherwise)
 template <typename real_t, int NH, bool IMG>
-__device__ __forceinline__ void neighbours(LocalArgs<real_t> const& a, int d, uint64_t blk, int p,
+__device__ __forceinline__ void neighbours(LocalArgs<real_t> const& a, int d, ItemHead<NH> const& h, int p,
                                            real_t const* (&nb)[2][NH], real_t (&fr)[2][NH], real_t (&fi)[2][NH]) {
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
         for (int k = 0; k < NH; ++k) {
-            int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+            int32_t const code = h.code(d, sd, k);
             uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
             nb[sd][k] = (code == -1) ? a.zero : a.x + idx * NBLK;
             bool const img = IMG && (code <= -2);
@@ -789,36 +805,32 @@ __device__ __forceinline__ void neighbours(LocalArgs<real_t> const& a, int d, ui
         }
 }
 
-// (V - E) x at the thread's 16 points (z plane zq of one column, plane p; zq uniform within the wave): own = x of
-// plane p there, xso = x of the other plane in LDS (row stride JW), the imaginary part of E coupling the two
-// planes; the centre weights of the three finite-difference directions are applied here as well
+// (V - E) x at the thread's 16 points (z plane zq of one column, plane p): own = x of plane p there, xso = x of the
+// other plane in LDS (row stride JW), vz = the potential at the 16 points (0 where the block has no potential
+// cell), E the column group's energy, whose imaginary part couples the two planes; the centre weights of the
+// three finite-difference directions are applied here as well
 template <typename real_t, int JW>
-__device__ __forceinline__ void plane_potential(LocalArgs<real_t> const& a, uint64_t blk, int p, int zq,
-                                                real_t const (&own)[16], real_t const* xso, real_t (&acc)[16]) {
-    int32_t const iv = a.veff[blk];
-    real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
-    // zq is uniform within the wave: the 16 potential values are scalar loads
-    double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + __builtin_amdgcn_readfirstlane(zq) * 16;
-    uint32_t const colE = a.colindx[blk];
-    real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
-    real_t const Vim = E_im * real_t(1 - 2 * p);
+__device__ __forceinline__ void plane_potential(LocalArgs<real_t> const& a, double const (&vz)[16], double E_re,
+                                                double E_im, int p, int zq, real_t const (&own)[16],
+                                                real_t const* xso, real_t (&acc)[16]) {
+    real_t const Vim = real_t(E_im) * real_t(1 - 2 * p);
+    real_t const dg = a.cdiag - real_t(E_re);
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
-        acc[k] = (real_t(Vb[k]) * vm + (a.cdiag - E_re)) * own[k] + Vim * xso[(zq * 16 + k) * JW];
+    for (int k = 0; k < 16; ++k) acc[k] = (real_t(vz[k]) + dg) * own[k] + Vim * xso[(zq * 16 + k) * JW];
 }
 
 // kinetic x and y at the thread's 16 points: the lines of plane zq along x (the rows y) and along y (the columns
 // x) through this block and NH neighbour blocks on each side, the neighbours straight from global memory.  zq is
 // uniform within the wave, so the plane's offset goes into the (scalar) base pointers.
 template <typename real_t, int NH, bool IMG>
-__device__ __forceinline__ void plane_xy(LocalArgs<real_t> const& a, uint64_t blk, int p, int zq, uint32_t boff_p,
-                                         uint32_t boff_o, real_t const (&own)[16], real_t (&acc)[16]) {
+__device__ __forceinline__ void plane_xy(LocalArgs<real_t> const& a, ItemHead<NH> const& h, int p, int zq,
+                                         uint32_t boff_p, uint32_t boff_o, real_t const (&own)[16], real_t (&acc)[16]) {
     constexpr int NW = 4 * (2 * NH + 1);
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
         real_t const* nb[2][NH];
         real_t fr[2][NH], fi[2][NH];
-        neighbours<real_t, NH, IMG>(a, d, blk, p, nb, fr, fi);
+        neighbours<real_t, NH, IMG>(a, d, h, p, nb, fr, fi);
         int const zqu = __builtin_amdgcn_readfirstlane(zq);
         real_t const* nbz[2][NH];
 #pragma unroll
@@ -849,31 +861,31 @@ __device__ __forceinline__ void plane_xy(LocalArgs<real_t> const& a, uint64_t bl
 }
 
 // The end of a register-plane kernel for one block slice: the SHO add-back of the atom images (numax <= 5)
-// touching blk, the store of the thread's 16 points and, if a.dot is set, the bilinear product (x, y) of every
-// column of the slice.  cs is LDS of at least 2 * 64 * JW elements that no thread reads any more (the
-// coefficients of one image at a time, then the partial sums of the product); xo = x of the other plane at the
-// thread's points.  Contains work-group barriers: every thread of the work-group calls it.
+// touching the block (add-list entries b0 .. b1 - 1, column group col), the store of the thread's 16 points and,
+// if a.dot is set, the bilinear product (x, y) of every column of the slice.  cs is LDS of at least 2 * 64 * JW
+// elements that no thread reads any more (the coefficients of one image at a time, then the partial sums of the
+// product); xo = x of the other plane at the thread's points.  Contains work-group barriers: every thread of the
+// work-group calls it.
 template <typename real_t, int JW>
-__device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_t blk, int slice, int p, int zq,
-                                             int jl, real_t (&acc)[16], real_t const (&own)[16],
-                                             real_t const (&xo)[16], real_t* cs, real_t (*H)[3][4], float (*xi2)[4]) {
+__device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_t blk, uint32_t col, uint32_t b0,
+                                             uint32_t b1, int slice, int p, int zq, int jl, real_t (&acc)[16],
+                                             real_t const (&own)[16], real_t const (&xo)[16], real_t* cs,
+                                             real_t (*H)[3][4], float (*xi2)[4]) {
     constexpr int T = 8 * JW;
     using vec_t = typename std::conditional<sizeof(real_t) == 8, double2, float4>::type;
     constexpr int VE = 16 / sizeof(real_t), VPR = JW / VE;
     int const tid = threadIdx.x, j = slice * JW + jl;
-    uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
     if (a.add_local && b1 > b0) {
-        uint32_t const col = a.colindx[blk];
         float const R2 = float(a.hgrid[3] * a.hgrid[3]);
         for (uint32_t b = b0; b < b1; ++b) {
-            uint32_t const img = a.add_index[b];
-            int const L = a.image_lmax[img];
+            AddEntry const e = a.add_entry[b];
+            int const L = int(e.L);
             if (L > 5) continue;  // done by k_sho_add_wide
             int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
             __syncthreads();
-            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
+            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, e.slot, L, tid, T);
             {   // [coefficient][plane][JW]
-                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
+                real_t const* src = a.dcoef + (size_t(e.coef0) * a.nCols + col) * 128 + slice * JW;
                 for (int v = tid; v < 2 * nc * VPR; v += T) {
                     int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
                     *reinterpret_cast<vec_t*>(cs + row * JW + cv * VE) =
@@ -934,7 +946,8 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     uint64_t const item = lin % a.nitems;
     int const slice = int(lin / a.nitems);
     if (slice >= NSLICE) return;  // uniform over the work-group
-    uint64_t const blk = a.order[item];
+    ItemHead<NH> const h = reinterpret_cast<ItemHead<NH> const*>(a.items)[item];   // one scalar load
+    uint64_t const blk = h.w[IH_BLK];
     int const tid = threadIdx.x;
 
     using vec_t = typename std::conditional<sizeof(real_t) == 8, double2, float4>::type;
@@ -947,9 +960,22 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
             *reinterpret_cast<vec_t*>(xs + row * JW + cv * VE) = *reinterpret_cast<vec_t const*>(src + row * 64 + cv * VE);
         }
     }
+    int const jl = tid % JW, p = (tid / JW) & 1, zq = tid / (2 * JW);
+    // the potential at the thread's plane and the energy, loaded (scalar: zq is uniform within a wave) together
+    // with the slice
+    double vz[16];
+    {
+        int32_t const iv = int32_t(h.w[IH_VEFF]);
+        double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + __builtin_amdgcn_readfirstlane(zq) * 16;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            double const v = Vb[k];   // in bounds also without a cell (row 0)
+            vz[k] = (iv >= 0) ? v : 0.0;
+        }
+    }
+    double const E_re = a.E_col[2 * h.w[IH_COL]], E_im = a.E_col[2 * h.w[IH_COL] + 1];
     __syncthreads();
 
-    int const jl = tid % JW, p = (tid / JW) & 1, zq = tid / (2 * JW);
     int const j = slice * JW + jl;
     real_t const* xsp = xs + p * 64 * JW + jl;
     real_t const* xso = xs + (1 - p) * 64 * JW + jl;
@@ -960,12 +986,12 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) own[k] = xsp[(zq * 16 + k) * JW];
 
-    plane_potential<real_t, JW>(a, blk, p, zq, own, xso, acc);
-    plane_xy<real_t, NH, IMG>(a, blk, p, zq, boff_p, boff_o, own, acc);
+    plane_potential<real_t, JW>(a, vz, E_re, E_im, p, zq, own, xso, acc);
+    plane_xy<real_t, NH, IMG>(a, h, p, zq, boff_p, boff_o, own, acc);
     {
         real_t const* nb[2][NH];
         real_t fr[2][NH], fi[2][NH];
-        neighbours<real_t, NH, IMG>(a, 2, blk, p, nb, fr, fi);
+        neighbours<real_t, NH, IMG>(a, 2, h, p, nb, fr, fi);
         switch (zq) {
             case 0: z_pass<real_t, NH, IMG, JW, 0>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
             case 1: z_pass<real_t, NH, IMG, JW, 1>(a, acc, own, xsp, nb, fr, fi, boff_p, boff_o); break;
@@ -981,7 +1007,8 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) xo[k] = xso[(zq * 16 + k) * JW];
     }
-    plane_finish<real_t, JW>(a, blk, slice, p, zq, jl, acc, own, xo, xs, H, xi2);
+    plane_finish<real_t, JW>(a, blk, h.w[IH_COL], h.w[IH_ADD0], h.w[IH_ADD1], slice, p, zq, jl, acc, own, xo, xs, H,
+                             xi2);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1103,15 +1130,15 @@ __device__ __forceinline__ void sho_xline(Lane<real_t> (&acc)[4], real_t const (
 // first column j0, into w; a periodic-image neighbour (uniform over the work-group) gets its Bloch phase,
 // f x_p - (+/-) g x_other, which waits for its loads (image blocks are few)
 template <typename real_t, int NH, bool IMG>
-__device__ __forceinline__ void load_lines(LocalArgs<real_t> const& a, int d, uint64_t blk, int p, int i0, int j0,
-                                           Lane<real_t> (&w)[2][NH][4]) {
+__device__ __forceinline__ void load_lines(LocalArgs<real_t> const& a, int d, ItemHead<NH> const& h, int p, int i0,
+                                           int j0, Lane<real_t> (&w)[2][NH][4]) {
     int const stride = (d == 0) ? 1 : ((d == 1) ? 4 : 16);
     uint32_t const off = uint32_t(p * 4096 + i0 * 64 + j0) * sizeof(real_t);
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
         for (int k = 0; k < NH; ++k) {
-            int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+            int32_t const code = h.code(d, sd, k);
             uint64_t const idx = (code >= 0) ? uint64_t(code) : uint64_t(code <= -2 ? -code - 2 : 0);
             real_t const* nb = (code == -1) ? a.zero : a.x + idx * NBLK;
 #pragma unroll
@@ -1123,7 +1150,7 @@ __device__ __forceinline__ void load_lines(LocalArgs<real_t> const& a, int d, ui
         for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
             for (int k = 0; k < NH; ++k) {
-                int32_t const code = a.nbr[((d * 2 + sd) * NH + k) * a.nnzb + blk];
+                int32_t const code = h.code(d, sd, k);
                 if (code <= -2) {
                     real_t const* nb = a.x + uint64_t(-code - 2) * NBLK;
                     real_t const fr = real_t(a.phase[d][sd][0]), fi = real_t(a.phase[d][sd][1]) * real_t(1 - 2 * p);
@@ -1155,7 +1182,8 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
     uint64_t const item = lin % a.nitems;   // slice-major, as in k_plane
     int const slice = int(lin / a.nitems);
     if (slice >= NSLICE) return;  // uniform over the work-group
-    uint64_t const blk = a.order[item];
+    ItemHead<NH> const h = reinterpret_cast<ItemHead<NH> const*>(a.items)[item];   // one scalar load
+    uint64_t const blk = h.w[IH_BLK];
     int const tid = threadIdx.x, cq = tid % LQ, p = (tid / LQ) & 1, l = tid / (2 * LQ);
     int const la = l & 3, lb = l >> 2;
     int const jc = cq * CPL, j0 = slice * JW + jc;
@@ -1163,9 +1191,9 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
     int const ix0 = lb * 16 + la * 4, iy0 = lb * 16 + la, iz0 = lb * 4 + la;
 
     L16 wx[2][NH][4], wy[2][NH][4], wz[2][NH][4];   // neighbour lines per direction
-    load_lines<real_t, NH, IMG>(a, 0, blk, p, ix0, j0, wx);
-    if constexpr (PF >= 2) load_lines<real_t, NH, IMG>(a, 1, blk, p, iy0, j0, wy);
-    if constexpr (PF >= 3) load_lines<real_t, NH, IMG>(a, 2, blk, p, iz0, j0, wz);
+    load_lines<real_t, NH, IMG>(a, 0, h, p, ix0, j0, wx);
+    if constexpr (PF >= 2) load_lines<real_t, NH, IMG>(a, 1, h, p, iy0, j0, wy);
+    if constexpr (PF >= 3) load_lines<real_t, NH, IMG>(a, 2, h, p, iz0, j0, wz);
     // own x-line, to registers and to the LDS copy of the slice
     L16 own[4];
     {
@@ -1183,10 +1211,10 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
         L16 oth[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) oth[q] = ld16(xs + ((1 - p) * 64 + ix0 + q) * JW + jc);
-        int32_t const iv = a.veff[blk];
+        int32_t const iv = int32_t(h.w[IH_VEFF]);
         real_t const vm = (iv >= 0) ? real_t(1) : real_t(0);
         double const* Vb = a.V + uint64_t(iv >= 0 ? iv : 0) * 64 + ix0;
-        uint32_t const colE = a.colindx[blk];
+        uint32_t const colE = h.w[IH_COL];
         real_t const E_re = real_t(a.E_col[2 * colE]), E_im = real_t(a.E_col[2 * colE + 1]);
         real_t const Vim = E_im * real_t(1 - 2 * p);
 #pragma unroll
@@ -1197,8 +1225,8 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
         }
         line_stencil<real_t, NH>(acc, a, 0, own, wx);
     }
-    if constexpr (PF < 2) load_lines<real_t, NH, IMG>(a, 1, blk, p, iy0, j0, wy);
-    if constexpr (PF == 2) load_lines<real_t, NH, IMG>(a, 2, blk, p, iz0, j0, wz);
+    if constexpr (PF < 2) load_lines<real_t, NH, IMG>(a, 1, h, p, iy0, j0, wy);
+    if constexpr (PF == 2) load_lines<real_t, NH, IMG>(a, 2, h, p, iz0, j0, wz);
 
     // ---- y stencil on the y-line, into ya ----
     {
@@ -1213,7 +1241,7 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
 #pragma unroll
         for (int q = 0; q < 4; ++q) st16(ya + (p * 64 + iy0 + 4 * q) * JW + jc, s[q]);
     }
-    if constexpr (PF < 2) load_lines<real_t, NH, IMG>(a, 2, blk, p, iz0, j0, wz);
+    if constexpr (PF < 2) load_lines<real_t, NH, IMG>(a, 2, h, p, iz0, j0, wz);
     __syncthreads();
     // ---- z stencil on the z-line, added into ya ----
     {
@@ -1236,20 +1264,20 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
     }
 
     // ---- SHO add-back of the atom images (numax <= 5) touching the block; coefficients staged in ya ----
-    uint32_t const b0 = a.add_start[blk], b1 = a.add_start[blk + 1];
+    uint32_t const b0 = h.w[IH_ADD0], b1 = h.w[IH_ADD1];
     if (a.add_local && b1 > b0) {
-        uint32_t const col = a.colindx[blk];
+        uint32_t const col = h.w[IH_COL];
         float const R2 = float(a.hgrid[3] * a.hgrid[3]);
         constexpr int VPR = JW / CPL;
         for (uint32_t b = b0; b < b1; ++b) {
-            uint32_t const img = a.add_index[b];
-            int const L = a.image_lmax[img];
+            AddEntry const e = a.add_entry[b];
+            int const L = int(e.L);
             if (L > 5) continue;  // done by k_sho_add_wide
             int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
             __syncthreads();   // ya (the stencil sums, or the previous image) is no longer read
-            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, T);
+            stage_hermite<real_t>(H, xi2, a.htab, a.xtab, e.slot, L, tid, T);
             {   // [coefficient][plane][JW]
-                real_t const* src = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + slice * JW;
+                real_t const* src = a.dcoef + (size_t(e.coef0) * a.nCols + col) * 128 + slice * JW;
                 for (int v = tid; v < 2 * nc * VPR; v += T) {
                     int const row = v / VPR, cv = v % VPR;  // row = coefficient*2 + plane
                     st16(ya + row * JW + cv * CPL, ld16(src + size_t(row >> 1) * a.nCols * 128 + (row & 1) * 64 + cv * CPL));
@@ -1449,7 +1477,29 @@ int dupload(DevicePlan& d, T*& ptr, std::vector<T> const& v, std::string& err) {
     return 0;
 }
 
+// items[i] = heads[order[i]] for i < n, one word per thread
+__global__ __launch_bounds__(256) void k_gather_heads(uint32_t const* __restrict__ order, uint64_t n,
+                                                      uint32_t const* __restrict__ heads, uint32_t* __restrict__ items,
+                                                      int hw) {
+    uint64_t const t = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= n * uint64_t(hw)) return;
+    uint64_t const i = t / hw, w = t % hw;
+    items[t] = heads[uint64_t(order[i]) * hw + w];
+}
+
 }  // namespace
+
+int device_gather_heads(DevicePlan const& d, uint32_t const* order, uint64_t n, uint32_t* items, hipStream_t stream,
+                        std::string& err) {
+    if (n == 0) return 0;
+    uint64_t const words = n * uint64_t(d.head_words);
+    uint64_t const grid = (words + 255) / 256;
+    if (grid > 0x7fffffffull) { err = "too many blocks for one launch"; return 1; }
+    hipLaunchKernelGGL(k_gather_heads, dim3(uint32_t(grid)), dim3(256), 0, stream, order, n, d.heads, items,
+                       d.head_words);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 
 void device_free(DevicePlan& d) {
     for (void* p : d.allocations) (void)hipFree(p);
@@ -1551,6 +1601,24 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
         }
         if ((s = dupload(d, d.nbr, nb, err))) return s;
 
+        // the local kernels' block headers: block, column group, potential cell, add-list range, neighbour codes
+        int const HW = (NH == 1) ? 16 : 32;
+        d.head_words = HW;
+        std::vector<uint32_t> heads(size_t(HW) * p.nnzb, 0);
+        for (uint64_t b = 0; b < p.nnzb; ++b) {
+            uint32_t* h = &heads[size_t(HW) * b];
+            h[IH_BLK] = uint32_t(b);
+            h[IH_COL] = p.colindx[b];
+            h[IH_VEFF] = uint32_t(p.veff_index[b]);
+            h[IH_ADD0] = p.sho_add.start[b];
+            h[IH_ADD1] = p.sho_add.start[b + 1];
+            for (int c = 0; c < 6 * NH; ++c) h[IH_NBR + c] = uint32_t(nb[size_t(c) * p.nnzb + b]);
+        }
+        std::vector<uint32_t> items(heads.size());
+        for (uint64_t i = 0; i < p.nnzb; ++i)
+            std::copy_n(&heads[size_t(HW) * d.h_order[i]], HW, &items[size_t(HW) * i]);
+        if ((s = dupload(d, d.heads, heads, err))) return s;
+        if ((s = dupload(d, d.items, items, err))) return s;
     }
 
     // dyadic structures
@@ -1582,6 +1650,18 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
             }
         d.slot_map = std::move(slot_of);
         if ((s = dupload(d, d.add_slot, add_slot, err))) return s;
+    }
+    {   // add-list entries in one record each: image, numax, Hermite slot, first coefficient of the image
+        std::vector<uint32_t> entry(4 * p.sho_add.index.size());
+        for (uint64_t blk = 0; blk < p.nnzb; ++blk)
+            for (uint32_t k = p.sho_add.start[blk]; k < p.sho_add.start[blk + 1]; ++k) {
+                uint32_t const img = p.sho_add.index[k];
+                entry[4 * size_t(k) + 0] = img;
+                entry[4 * size_t(k) + 1] = uint32_t(p.image_lmax[img]);
+                entry[4 * size_t(k) + 2] = d.slot_map.at((uint64_t(img) << 32) | p.rowindx[blk]);
+                entry[4 * size_t(k) + 3] = p.image_starts[img];
+            }
+        if ((s = dupload(d, d.add_entry, entry, err))) return s;
     }
     if ((s = dupload(d, d.image_pos, p.image_pos, err))) return s;
     if ((s = dupload(d, d.image_lmax, p.image_lmax, err))) return s;
@@ -1890,7 +1970,8 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     a.row_pos = d.row_pos; a.image_pos = d.image_pos; a.image_lmax = d.image_lmax; a.image_starts = d.image_starts;
     a.nnzb = d.nnzb; a.nCols = d.nCols;
     a.zero = ws.zero;
-    a.order = d.active_order ? d.active_order : d.order;
+    a.items = d.active_order ? d.active_items : d.items;
+    a.add_entry = reinterpret_cast<AddEntry const*>(d.add_entry);
     a.nitems = d.active_order ? d.n_active : d.nnzb;
     a.E_col = d.E_col;
     a.htab = d.htab; a.xtab = d.xtab; a.add_slot = d.add_slot;

@@ -621,10 +621,13 @@ int gfa_solve_memory(gfa_plan* plan, const gfa_solve_opts* opts, uint64_t* solve
         if (plan_bytes) {
             if (plan->dev.uploaded) {
                 *plan_bytes = plan->dev.bytes;
-            } else {   // index data (about 100 bytes per block and 12 per add/projection entry) and the potential
+            } else {   // index data (about 100 bytes per block, the local kernels' block headers twice, 28 bytes
+                       // per add entry and 12 per projection entry) and the potential
                 gfa::Plan const& h = plan->host;
-                uint64_t b = h.nnzb * 120 + h.ncell() * 64 * 8 + h.atom_hs.size() * 8;
-                b += h.sho_add.index.size() * 12;
+                bool wide = false;
+                for (int d = 0; d < 3; ++d) wide = wide || h.kinetic[d].fd_range == 8;
+                uint64_t b = h.nnzb * (120 + 2 * 4 * (wide ? 32 : 16)) + h.ncell() * 64 * 8 + h.atom_hs.size() * 8;
+                b += h.sho_add.index.size() * 28;
                 for (auto const& c : h.sho_prj) b += c.index.size() * 12;
                 b += uint64_t(h.natomcoeffs()) * h.nCols * 128 * 8 * 2;   // coefficient workspace (both precisions)
                 *plan_bytes = b;

@@ -63,6 +63,13 @@ struct DevicePlan {
     int32_t* veff = nullptr;           // [nnzb]
     int32_t* nbr = nullptr;            // [3][2][nh_max][nnzb] neighbour codes
     uint32_t* order = nullptr;         // [nnzb] processing order of k_local
+    // per-block headers of the local kernels, head_words (16, or 32 when nh_max = 2) words each: the block, its
+    // column group, potential cell, add-list range and neighbour codes [direction][side][k], so that a work-group
+    // reads everything it needs about its block with one scalar load
+    uint32_t* heads = nullptr;         // [nnzb][head_words] in block order
+    uint32_t* items = nullptr;         // [nnzb][head_words] in processing order
+    int head_words = 16;
+    uint32_t* add_entry = nullptr;     // [add nnz][4]: image, numax, Hermite slot, first coefficient of the image
     // Hermite functions per (atom image, target row) slot: htab[slot][8][3][4], xtab[slot][3][4] = xi^2
     double* htab = nullptr;
     float* xtab = nullptr;
@@ -104,6 +111,7 @@ struct DevicePlan {
     double* E_col = nullptr;           // [nCols][2] energy per column group
     // retirement: when set, the action processes only the listed blocks and skips inactive columns
     uint32_t* active_order = nullptr;  // [n_active] blocks of active column groups, in processing order
+    uint32_t* active_items = nullptr;  // [n_active][head_words] their headers
     uint64_t n_active = 0;
     uint8_t* col_active = nullptr;     // [nCols] 1 for an active column group
     uint64_t* atom_mat_off = nullptr;  // [nAtoms+1]
@@ -123,6 +131,10 @@ struct ActionScalars {
     double prefactor[3];
     double hgrid[4];                  // grid spacings and projection radius (units of sigma)
 };
+
+// headers (heads[order[i]]) of n blocks into items, on the stream
+int device_gather_heads(DevicePlan const& d, uint32_t const* order, uint64_t n, uint32_t* items, hipStream_t stream,
+                        std::string& err);
 
 // Upload the index data (once) and the per-(E,k) data; 0 or error.
 int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err);

@@ -932,6 +932,7 @@ struct SolverState {
     uint32_t nseg = 0;                      // active segments
     uint32_t* act_seg = nullptr;            // [nseg_all] global numbers of the active segments
     uint32_t* act_order = nullptr;          // [nnzb] blocks of active groups in the action's order
+    uint32_t* act_items = nullptr;          // [nnzb][head_words] their local-kernel headers
     uint64_t n_act_blocks = 0;
     uint8_t* col_active = nullptr;          // [nCols]
     std::vector<uint32_t> h_act_seg, h_act;
@@ -1033,6 +1034,7 @@ int setup(gfa_plan* plan, SolverState& st, std::string& err) {
     if ((s = walloc(st, st.col_seg, size_t(h.nCols) + 1, err))) return s;
     if ((s = walloc(st, st.act_seg, st.nseg_all, err))) return s;
     if ((s = walloc(st, st.act_order, h.nnzb, err))) return s;
+    if ((s = walloc(st, st.act_items, h.nnzb * uint64_t(capi_dev(plan).head_words), err))) return s;
     if ((s = walloc(st, st.col_active, h.nCols, err))) return s;
     HIP_TRY(hipMemcpy(st.col_blocks, st.h_cb.data(), h.nnzb * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(st.diag, diag.data(), h.nnzb, hipMemcpyHostToDevice));
@@ -1066,8 +1068,10 @@ int set_active(gfa_plan* plan, SolverState& st, std::vector<uint8_t> const& acti
     for (uint32_t b : dev.h_order) if (active[h.colindx[b]]) st.h_act.push_back(b);
     st.n_act_blocks = st.h_act.size();
     if (st.nseg) HIP_TRY(hipMemcpyAsync(st.act_seg, st.h_act_seg.data(), st.nseg * 4, hipMemcpyHostToDevice, st.stream));
-    if (st.n_act_blocks)
+    if (st.n_act_blocks) {
         HIP_TRY(hipMemcpyAsync(st.act_order, st.h_act.data(), st.n_act_blocks * 4, hipMemcpyHostToDevice, st.stream));
+        if (int r = device_gather_heads(dev, st.act_order, st.n_act_blocks, st.act_items, st.stream, err)) return r;
+    }
     HIP_TRY(hipMemcpyAsync(st.col_active, active.data(), st.nCols, hipMemcpyHostToDevice, st.stream));
     HIP_TRY(hipStreamSynchronize(st.stream));  // the host vectors may change before the copies ran otherwise
     ++st.host_syncs;
@@ -1079,9 +1083,10 @@ int set_active(gfa_plan* plan, SolverState& st, std::vector<uint8_t> const& acti
 struct ActiveScope {
     DevicePlan& d;
     ActiveScope(DevicePlan& d_, SolverState const& st) : d(d_) {
-        d.active_order = st.act_order; d.n_active = st.n_act_blocks; d.col_active = st.col_active;
+        d.active_order = st.act_order; d.active_items = st.act_items; d.n_active = st.n_act_blocks;
+        d.col_active = st.col_active;
     }
-    ~ActiveScope() { d.active_order = nullptr; d.n_active = 0; d.col_active = nullptr; }
+    ~ActiveScope() { d.active_order = nullptr; d.active_items = nullptr; d.n_active = 0; d.col_active = nullptr; }
 };
 
 // work vector k of precision T in the arena
@@ -1189,7 +1194,7 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
         if (!any) break;
         if (changed && (capped || 8 * live_blocks < 7 * st.n_act_blocks)) {
             if (int r = set_active(plan, st, act, err)) return r;
-            scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
+            scope.d.active_order = st.act_order; scope.d.active_items = st.act_items; scope.d.n_active = st.n_act_blocks;
         }
     }
     {   // per-column half-step counts of this run (columns of inactive groups count 0)
@@ -1318,7 +1323,7 @@ int cocg(gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, d
     if (rc) return rc;
     if (f32) {   // the increments of every group that took part, also of those retired since the last fold
         if (int e = set_active(plan, st, started, err)) return e;
-        scope.d.active_order = st.act_order; scope.d.n_active = st.n_act_blocks;
+        scope.d.active_order = st.act_order; scope.d.active_items = st.act_items; scope.d.n_active = st.n_act_blocks;
         if (int e = fold()) return e;
     }
     return 0;
@@ -1368,6 +1373,11 @@ uint64_t solve_bytes(gfa_plan* plan, int fp_bits, int method) {
     uint64_t const vec64 = nnzb * NBLK * 8;
     uint64_t bytes = 2 * vec64;                          // x64, y64
     bytes += nnzb * (4 + 1 + 4) + nnzb * 128 * 8;        // col_blocks, diag, act_order; the per-block (p, A p)
+    {   // the active blocks' headers: 16 words, 32 with the 16th-order stencil
+        bool wide = false;
+        for (int d = 0; d < 3; ++d) wide = wide || h.kinetic[d].fd_range == 8;
+        bytes += nnzb * 4 * uint64_t(wide ? 32 : 16);
+    }
     bytes += nseg * (12 + 4) + nseg * 3 * 64 * 8;        // segments, active segments, segment partials
     bytes += ncol * 8 * (NFIELDS + 2) + uint64_t(h.nCols) * 13;
     int const m = method == 1 ? 1 : 0;
