@@ -379,6 +379,19 @@ __global__ __launch_bounds__(64 * ATOM_WAVES) void k_sho_atoms(
 //    branch-free addressing (an absent neighbour reads a zero block) so that all loads of a line are in flight
 //    together.
 // ---------------------------------------------------------------------------------------------
+// 16 bytes of y, stored past the caches' replacement order (non-temporal): the output of the local kernels is read
+// by the next kernel only, so keeping it out of L2 leaves room there for the neighbour blocks of x
+__device__ __forceinline__ void st_y16(float* p, float4 const& v) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    f4v t; t[0] = v.x; t[1] = v.y; t[2] = v.z; t[3] = v.w;
+    __builtin_nontemporal_store(t, reinterpret_cast<f4v*>(p));
+}
+__device__ __forceinline__ void st_y16(double* p, double2 const& v) {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    d2v t; t[0] = v.x; t[1] = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+}
+
 // word indices in a block header: block, column group, potential cell (-1: none), add-list range [ADD0, ADD1),
 // then the neighbour codes [direction][side][k < NH] (>= 0 block, -1 absent, <= -2 periodic image -code - 2)
 enum : int { IH_BLK = 0, IH_COL = 1, IH_VEFF = 2, IH_ADD0 = 3, IH_ADD1 = 4, IH_NBR = 5 };
@@ -704,7 +717,7 @@ __global__ __launch_bounds__(8 * JW) void k_local(LocalArgs<real_t> a) {
 #pragma unroll
         for (int k = 0; k < 2 * 64 * VPR / T; ++k) {
             int const v = k * T + tid, row = v / VPR, cv = v % VPR;
-            *reinterpret_cast<vec_t*>(dst + row * 64 + cv * VE) = *reinterpret_cast<vec_t const*>(ys + row * JW + cv * VE);
+            st_y16(dst + row * 64 + cv * VE, *reinterpret_cast<vec_t const*>(ys + row * JW + cv * VE));
         }
     }
     if (a.dot) {   // (x, y) = sum_i x_i y_i (no conjugation) per column, x re-read from global memory (L2)
@@ -899,7 +912,7 @@ __device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_
 
     real_t* dst = a.y + blk * NBLK + size_t(p * 4096 + j) + size_t(zq * 16) * 64;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) dst[size_t(k) * 64] = acc[k];
+    for (int k = 0; k < 16; ++k) __builtin_nontemporal_store(acc[k], &dst[size_t(k) * 64]);   // y is not re-read here
     if (a.dot) {
         // (x, y) = sum_i x_i y_i per column: plane 0 holds x_r, y_r (+ x_i from LDS), plane 1 x_i, y_i (+ x_r):
         // Re = sum x_r y_r - sum x_i y_i, Im = sum x_i y_r + sum x_r y_i
@@ -1039,6 +1052,11 @@ __device__ __forceinline__ void st16(float* p, Lane<float> const& v) {
 }
 __device__ __forceinline__ void st16(double* p, Lane<double> const& v) {
     *reinterpret_cast<double2*>(p) = double2{v.v[0], v.v[1]};
+}
+template <typename real_t>
+__device__ __forceinline__ void st16_nt(real_t* p, Lane<real_t> const& v) {   // non-temporal, for y
+    if constexpr (sizeof(real_t) == 4) st_y16(p, float4{v.v[0], v.v[1], v.v[2], v.v[3]});
+    else st_y16(p, double2{v.v[0], v.v[1]});
 }
 // 16 bytes at element e of the uniform base b plus the lane's byte offset off (the load's own addressing mode)
 template <typename real_t>
@@ -1300,7 +1318,7 @@ __global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs
     {
         real_t* dst = a.y + blk * NBLK + p * 4096 + j0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) st16(dst + (ix0 + q) * 64, acc[q]);
+        for (int q = 0; q < 4; ++q) st16_nt(dst + (ix0 + q) * 64, acc[q]);
     }
     if (a.dot) {
         // (x, y) = sum_i x_i y_i per column: plane 0 holds x_r, y_r (+ x_i), plane 1 x_i, y_i (+ x_r):
