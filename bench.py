@@ -39,6 +39,10 @@ FP64_SHAPES = [(0, 16, 2), (1, 32, 0), (2, 16, 3), (2, 32, 3), (2, 16, 2), (2, 3
 # HBM bytes per launch of the FP32 local kernel per tile shape on this workload (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE of --mode action, scaled by a plain copy's counts); the command that made them is in the file
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "local_fp32_contour32_traffic.json")
+# MFMA utilisation of the matrix-core forms of the SHO projection and add-back on this workload (rocprofv3
+# SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU_MFMA_MOPS_* of --mode dyadic); the commands are in the file
+MFMA_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "dyadic_mfma_contour32_counters.json")
+KERNEL_OF_KIND = {0: "k_local", 1: "k_plane", 2: "k_lines"}
 SOURCE = (16, 16, 16)
 NBATCH = 4
 
@@ -157,18 +161,24 @@ def main():
         batch_mode(G, sysm, instances, n, args, opts)
         return
 
-    # ---- warmup (the first one also tunes the tile shapes of both precisions) ----
-    shapes = None
+    # ---- warmup (the first one also tunes the tile shapes and dyadic engines of both precisions) ----
+    shapes, engines = None, None
+
+    def pin(p, fps=(32, 64)):   # the tuned local-kernel shapes and dyadic engines
+        if shapes:
+            for fp in fps:
+                p.set_tile_shape(fp, *shapes[fp])
+                p.set_dyadic_engine(fp, engines[fp])
+
     for w in range(args.warmup):
         p = batch_plan(G, sysm, batch(instances, w, n))
-        if shapes:
-            for fp in (32, 64):
-                p.set_tile_shape(fp, *shapes[fp])
+        pin(p)
         _, st = p.solve(fp_bits=32, autotune=True, **opts)
         if shapes is None:
             shapes = {fp: p.tile_shape(fp)[:3] for fp in (32, 64)}
+            engines = {fp: p.dyadic_engine(fp) for fp in (32, 64)}
         log(f"# warmup {w}: {st['converged_columns']}/{st['total_columns']} columns, {st['seconds']:.2f} s, "
-            f"tile shapes {shapes}")
+            f"tile shapes {shapes}, dyadic engines {engines}")
         p.close()
 
     # ---- the measured path: batched mixed-precision solves, one plan per step ----
@@ -183,9 +193,7 @@ def main():
         b = k % NBATCH
         inst = batch(instances, k, n)
         p = batch_plan(G, sysm, inst)   # untimed: host plan, device upload, workspaces, pinned tile shapes
-        if shapes:
-            for fp in (32, 64):
-                p.set_tile_shape(fp, *shapes[fp])
+        pin(p)
         p.prepare(32)
         p.prepare(64)
         keep = (b not in errors) and not args.no_fp64
@@ -237,8 +245,7 @@ def main():
         cols, t_seq, hs_seq, nb_seq = 0, 0.0, [], []
         for d in sample:
             p1 = batch_plan(G, sysm, [d])
-            if shapes:
-                p1.set_tile_shape(64, *shapes[64])
+            pin(p1, (64,))
             p1.prepare(64)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
@@ -256,9 +263,7 @@ def main():
         # batch 0 with the other methods, while the wall-clock budget allows
         if time.time() - t_begin < args.budget_s:
             p = batch_plan(G, sysm, first)
-            for fp in (32, 64):
-                if shapes:
-                    p.set_tile_shape(fp, *shapes[fp])
+            pin(p)
             p.prepare(32)
             p.prepare(64)
             for name, kw in (("tfqmr_batched_columns_per_s",
@@ -280,9 +285,7 @@ def main():
 
     # ---- roofline of the local kernel (potential + kinetic + SHO add-back), HIP events on its stream ----
     plan = batch_plan(G, sysm, first)
-    if shapes:
-        for fp in (32, 64):
-            plan.set_tile_shape(fp, *shapes[fp])
+    pin(plan)
     nnzb = plan.nnzb
     t_loc64 = time_action(plan, 64, 2, args.action_reps, stream)
     t_loc32 = time_action(plan, 32, 2, args.action_reps, stream)
@@ -292,6 +295,15 @@ def main():
     nrows = plan.nRows
     alg64 = nnzb * 2 * BLOCK_BYTES[64] + nrows * 64 * 8
     alg32 = nnzb * 2 * BLOCK_BYTES[32] + nrows * 64 * 8
+    # the whole action with the dyadic term on each engine: vector ALUs (sum-factorised) and matrix cores (dense)
+    engine_ms = {}
+    for fp in (32, 64):
+        used = plan.dyadic_engine(fp)
+        for eng in (0, 1):
+            plan.set_dyadic_engine(fp, eng)
+            engine_ms[f"{('vector', 'matrix')[eng]}_fp{fp}"] = round(
+                time_action(plan, fp, 3, args.action_reps, stream) * 1e3, 3)
+        plan.set_dyadic_engine(fp, used)
     plan.close()
 
     cpu = None
@@ -328,10 +340,14 @@ def main():
                    "fp64_GBs": round(alg64 / t_act64 / 1e9, 1), "fp32_GBs": round(alg32 / t_act32 / 1e9, 1),
                    "nnzb_batch0": int(nnzb),
                    "local_fp64_frac": round(alg64 / t_loc64 / 1e9 / HBM_PEAK_GBS, 4)},
+        "dyadic_engines": {"used": {f"fp{fp}": ("vector", "matrix")[engines[fp]] for fp in (32, 64)} if engines
+                           else None,
+                           "action_ms": engine_ms,
+                           "mfma": mfma_of(nnzb)},
         "roofline": {"bound": "hbm", "achieved": round(alg32 / t_loc32 / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg32 / t_loc32 / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": traffic_of(shape32, nnzb),
-                     "kernel": f"FP32 local kernel, tile shape {shape32} ({'k_local' if shape32[0] == 0 else 'k_plane'}: "
+                     "kernel": f"FP32 local kernel, tile shape {shape32} ({KERNEL_OF_KIND.get(shape32[0])}: "
                                "potential + 8th-order kinetic + SHO add-back), the one the mixed solve ran, on the "
                                "batch-0 plan; algorithmic bytes per launch = nnzb * 2 * 32 KiB + nRows * 512 B"},
         "cpu_baseline": cpu,
@@ -356,6 +372,8 @@ def action_mode(G, sysm, instances, n, args, stream, dev):
         p.set_tile_shape(64, *shape)
         t = time_action(p, 64, 2, args.action_reps, stream)
         log(f"# local fp64 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
+    p.set_tile_shape(32, 1, 32, 0)   # the default shapes again for the engine comparison
+    p.set_tile_shape(64, 0, 16, 2)
     dyadic_engines(p, args, stream)
     # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
     x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
@@ -469,6 +487,20 @@ def traffic_of(shape, nnzb):
     if t.get("nnzb") != int(nnzb) or e is None:
         return None
     return int(e["fetch_bytes"] + e["write_bytes"])
+
+
+def mfma_of(nnzb):
+    """MFMA utilisation and rate of the matrix-core SHO kernels from the committed rocprofv3 counter summary of the
+    same workload; None if it covers another workload."""
+    try:
+        with open(MFMA_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("nnzb") != int(nnzb):
+        return None
+    return {k: {"mfma_util": e["mfma_util"], "TFLOPs": e["tflops"], "peak_TFLOPs": e["peak_tflops"]}
+            for k, e in t.get("kernels", {}).items()}
 
 
 def cpu_baseline(G, sysm, sample, half_steps, args):
