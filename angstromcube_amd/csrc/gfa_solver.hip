@@ -1448,6 +1448,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     else st.group_cap.clear();
     std::vector<uint8_t> all(st.nCols, 1);
     st.col_halfsteps.assign(st.ncol, 0.0);
+    int fp64_groups = 0;   // FP32 solves: column groups that went on in FP64
     if (int r = set_active(plan, st, all, err)) return r;
     {
         uint64_t const n = st.nnzb * NBLK;
@@ -1475,10 +1476,13 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             if (int r = fetch_resid()) return r;
         }
     } else {
-        // FP32 inner solves inside FP64 refinement, per column group: a group stays in FP32 while each FP32 solve
-        // reduces its true (FP64) residual at least tenfold; a group whose FP32 solve gains less (the FP32
-        // recurrences of an ill-conditioned instance stall) is solved again in FP64 from zero.  The FP64 work
-        // vectors take the arena of the FP32 ones (the two never run at the same time).
+        // FP32 inner solves inside FP64 refinement, per column group.  tfQMR: a group stays in FP32 while each FP32
+        // solve reduces its true (FP64) residual at least tenfold; a group whose FP32 solve gains less (the FP32
+        // recurrences of an ill-conditioned instance stall) is solved again in FP64 from zero.  COCG: a group that
+        // spends the FP32 budget without converging continues in FP64 from its FP32 iterate (COCG with residual
+        // replacement does not stall, but in FP32 it needs up to several times the FP64 iterations on
+        // ill-conditioned instances, and an FP64 iteration costs about twice an FP32 one).  The FP64 work vectors
+        // take the arena of the FP32 ones (the two never run at the same time).
         double const floor_rel = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;  // gain asked of one FP32 solve
         int const budget32 = o.fp32_max_iterations > 0 ? o.fp32_max_iterations : o.max_iterations;
         std::vector<uint8_t> prec64(st.nCols, 0), done(st.nCols, 0);
@@ -1495,6 +1499,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             if (!any32 && !any64) break;
             if (any32) {
                 if (int r = set_active(plan, st, a32, err)) return r;
+                std::vector<double> const hs0 = st.col_halfsteps;   // applications before this FP32 solve
                 TfqmrResult tr;
                 if (int r = krylov<float>(o.method, plan, st, first32 ? nullptr : st.y64, floor_rel, st.tol_abs, budget32,
                                          check_every, seed + k, eager, o.autotune != 0, tr, err)) return r;
@@ -1507,9 +1512,12 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
                 for (uint32_t c = 0; c < st.nCols; ++c) {
                     if (!a32[c]) continue;
                     double const g = group_max(c);
+                    double used = 0;   // applications of this FP32 solve to the group
+                    for (int j = 0; j < 64; ++j) used = std::max(used, st.col_halfsteps[64 * c + j] - hs0[64 * c + j]);
                     mx = std::max(mx, g);
                     if (g <= o.tolerance) done[c] = 1;
-                    else if (g > 0.1 * gres[c]) { prec64[c] = 1; fresh64[c] = 1; }
+                    else if (o.method == 1 && used >= budget32) { prec64[c] = 1; ++fp64_groups; }
+                    else if (g > 0.1 * gres[c]) { prec64[c] = 1; fresh64[c] = 1; ++fp64_groups; }
                     gres[c] = g;
                 }
                 if (o.echo > 0)
@@ -1570,6 +1578,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
     ++st.host_syncs;
     S.host_syncs = st.host_syncs;
     S.graph_captures = st.graph_captures;
+    S.fp64_groups = fp64_groups;
     st.group_cap.clear();
     S.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     S.flops = double(S.actions) * reference_flops(capi_host(plan), o.fp_bits);

@@ -86,10 +86,10 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged_columns", C.c_int32),
                 ("total_columns", C.c_int32), ("max_residual", C.c_double), ("seconds", C.c_double),
                 ("flops", C.c_double), ("actions", C.c_int32), ("host_syncs", C.c_int32), ("graph_captures", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("fp64_groups", C.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None

@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=32, help="instances per batch (<= 32)")
     ap.add_argument("--maxit", type=int, default=8000)
-    ap.add_argument("--fp32-maxit", type=int, default=3000, help="operator applications allowed to one FP32 solve")
+    ap.add_argument("--fp32-maxit", type=int, default=0,
+                    help="operator applications allowed to one FP32 solve (0: 1200 for COCG, whose column groups then "
+                         "go on in FP64, 3000 for tfQMR)")
     ap.add_argument("--method", choices=["cocg", "tfqmr"], default="cocg",
                     help="inner Krylov method of the mixed solve")
     ap.add_argument("--inner-tol", type=float, default=0.0,
@@ -151,8 +153,9 @@ def main():
         return
 
     inner = args.inner_tol if args.inner_tol > 0 else (1e-30 if args.method == "cocg" else 1e-4)
+    fp32_budget = args.fp32_maxit or (1200 if args.method == "cocg" else 3000)
     opts = dict(tolerance=1e-9, max_iterations=args.maxit, max_refinements=6, inner_tolerance=inner,
-                fp32_max_iterations=args.fp32_maxit, method=args.method)
+                fp32_max_iterations=fp32_budget, method=args.method)
 
     if args.mode == "batch128":
         batch128_mode(G, sysm, instances, opts)
@@ -452,7 +455,7 @@ def contour_mode(G, sysm, args):
     kw = dict(nsources=(2, 2, 2), kpoints=[(0.0, 0.0, 0.0, 1.0)], E_Fermi=0.0, tolerance=1e-9)
     runs = {}
     for name, opts in (("batched_mixed_cocg", dict(max_groups=64, fp_bits=32, method="cocg", max_refinements=6,
-                                                    inner_tolerance=1e-30, fp32_max_iterations=3000,
+                                                    inner_tolerance=1e-30, fp32_max_iterations=1200,
                                                     max_iterations=8000, autotune=True)),
                        ("per_energy_fp64_tfqmr", dict(max_groups=8, fp_bits=64, method="tfqmr", max_refinements=2,
                                                       max_iterations=8000))):

@@ -202,9 +202,10 @@ typedef struct {
     int32_t echo;               /* >0: one line per inner solve on stderr                             */
     int32_t eager;              /* 0: replay each iteration from a captured HIP graph; 1: plain launches */
     int32_t autotune;           /* 1: time the action's tile shapes on the solver's vectors first (once per precision) */
-    int32_t fp32_max_iterations;/* fp_bits 32: half-steps allowed to one FP32 solve (0: max_iterations); a column
-                                 * group whose FP32 solve reduces its residual less than tenfold is solved again
-                                 * in FP64 from zero                                                          */
+    int32_t fp32_max_iterations;/* fp_bits 32: half-steps allowed to one FP32 solve (0: max_iterations).  tfQMR:
+                                 * a column group whose FP32 solve reduces its residual less than tenfold is
+                                 * solved again in FP64 from zero.  COCG: a group still unconverged when its FP32
+                                 * solve has used this budget continues in FP64 from the FP32 iterate            */
     double* residuals;          /* optional host output [64 nCols]: final relative residual of every column          */
     int32_t* halfsteps;         /* optional host output [64 nCols]: operator applications of every column (all solves) */
     int32_t method;             /* inner Krylov method: 0 tfQMR (as the reference), 1 COCG (conjugate orthogonal CG;
@@ -229,7 +230,7 @@ typedef struct {
     int32_t actions;             /* number of action applications                                    */
     int32_t host_syncs;          /* host waits on the device (status polls, residual fetches, set-ups)     */
     int32_t graph_captures;      /* HIP graph captures of a solver step (one per change of the active set) */
-    int32_t reserved;
+    int32_t fp64_groups;         /* fp_bits 32: column groups that went on in FP64 (see fp32_max_iterations) */
 } gfa_solve_stats;
 
 /* rho[nCols][64][2]: Green function diagonal G_ii / pi (re, im) as include/green_action.hxx:226-238;

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 SOURCE = (16, 16, 16)
 MIXED = dict(fp_bits=32, tolerance=1e-9, max_iterations=8000, max_refinements=6, inner_tolerance=1e-30,
-             fp32_max_iterations=3000, method="cocg")
+             fp32_max_iterations=1200, method="cocg")
 
 
 def batch_plan(sysm, inst):

@@ -139,6 +139,26 @@ def test_mixed_solve_reaches_fp64_tolerance(device, name, mk, kw, E, k):
 
 
 @pytest.mark.parametrize("name,mk,kw,E,k", [c for c in CASES if c[4] == (0.0, 0.0, 0.0)])
+def test_cocg_groups_over_the_fp32_budget_go_on_in_fp64(device, name, mk, kw, E, k):
+    """Mixed COCG whose FP32 budget (8 applications) runs out first: the unconverged column groups go on in FP64
+    from their FP32 iterate, and the solve still meets the residual bar (oracle action) and gives the FP64
+    solution."""
+    plan = G.GreenPlan(mk(), **kw)
+    plan.set_inputs(E, k)
+    X64, _, s64 = run(plan, device, fp_bits=64, tolerance=1e-9, max_iterations=2000, max_refinements=4, method="cocg")
+    Xb, _, sb = run(plan, device, fp_bits=32, tolerance=1e-9, max_iterations=2000, max_refinements=6, method="cocg",
+                    inner_tolerance=1e-30, fp32_max_iterations=8)
+    r = true_residuals(plan, O.Oracle(plan, E, k), Xb)
+    err = np.abs(Xb - X64).max() / np.abs(X64).max()
+    print(f"{name}: {sb['fp64_groups']} of {plan.nCols} groups went on in FP64, {sb['iterations']} applications "
+          f"(fp64 alone {s64['iterations']}), residual {r.max():.2e}, |X - X64| / |X64| = {err:.2e}")
+    assert sb["fp64_groups"] == plan.nCols
+    assert sb["converged_columns"] == sb["total_columns"]
+    assert r.max() <= 1e-9
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("name,mk,kw,E,k", [c for c in CASES if c[4] == (0.0, 0.0, 0.0)])
 def test_cocg_solves_complex_symmetric_systems(device, name, mk, kw, E, k):
     """COCG (Gamma: A^T = A) in FP64 and in FP32 with FP64 refinement reaches the tolerance, measured with the
     oracle's action, and gives the tfQMR solution."""
