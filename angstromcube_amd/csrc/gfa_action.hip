@@ -413,26 +413,16 @@ struct LocalArgs {
     real_t const* dcoef;
     real_t const* zero;      // one block of zeros
     uint32_t const* items;   // block headers in processing order (cache-aware), ItemHead<NH> each
-    AddEntry const* add_entry;
-    int32_t const* veff;
-    int32_t const* nbr;      // [3][2][NH][nnzb]
+    AddEntry const* add_entry;   // [add nnz]
     double const* V;
-    uint32_t const* add_start;
-    uint32_t const* add_index;
+    uint32_t const* add_start;   // [nnzb + 1] add-list range of each block (the add-back kernels after the local one)
     uint16_t const* colindx;
-    uint32_t const* rowindx;
-    float const* row_pos;
-    double const* image_pos;
-    int8_t const* image_lmax;
-    uint32_t const* image_starts;
-    uint64_t nnzb;
-    uint64_t nitems;         // blocks to process (all, or the active ones) listed in order[]
+    uint64_t nitems;         // blocks to process (all, or the active ones) listed in items
     uint32_t nCols;
     uint32_t per_xcd;        // work-items per XCD in the XCD-contiguous mapping
     double const* E_col;     // [nCols][2] energy of each column group
     double const* htab;      // Hermite functions per (image, row) slot
     float const* xtab;
-    uint32_t const* add_slot;  // slot of each add-list entry
     double* dot;             // if set: [nnzb][re, im][64] bilinear products sum_i x[b][i][j] y[b][i][j] per block
     int add_local;           // 1: the local kernel adds the numax <= 5 images; 0: k_sho_add_mfma adds them after it
     real_t c8[3][5];         // 8th-order weights per direction
@@ -1397,12 +1387,12 @@ __global__ __launch_bounds__(256) void k_sho_add_mfma(LocalArgs<real_t> a, uint3
         for (int t = 0; t < 2; ++t) acc[mt][t] = acc_t{0, 0, 0, 0};
     int const nb0 = 32 * w + (lane & 15);   // B operand column of tile 0 (tile 1: + 16)
     for (uint32_t b = a.add_start[blk]; b < a.add_start[blk + 1]; ++b) {
-        uint32_t const img = a.add_index[b];
-        int const L = a.image_lmax[img];
+        AddEntry const e = a.add_entry[b];
+        int const L = int(e.L);
         if (L > 5) continue;   // added by k_sho_add_wide
         int const nc = ((L + 1) * (L + 2) * (L + 3)) / 6;
         __syncthreads();   // P of the previous image is no longer read
-        stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, 256);
+        stage_hermite<real_t>(H, xi2, a.htab, a.xtab, e.slot, L, tid, 256);
         if (tid == 0) {
             int q = 0;
             for (int iz = 0; iz <= L; ++iz)
@@ -1421,7 +1411,7 @@ __global__ __launch_bounds__(256) void k_sho_add_mfma(LocalArgs<real_t> a, uint3
             P[e] = v;
         }
         __syncthreads();
-        real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + nb0;
+        real_t const* dsrc = a.dcoef + (size_t(e.coef0) * a.nCols + col) * 128 + nb0;
         size_t const dstride = size_t(a.nCols) * 128;
         int const nks = (nc + 3) / 4;
         for (int ks = 0; ks < nks; ++ks) {
@@ -1465,13 +1455,13 @@ __global__ __launch_bounds__(512) void k_sho_add_wide(LocalArgs<real_t> a, uint3
     float const R2 = float(a.hgrid[3] * a.hgrid[3]);
     real_t* yb = a.y + size_t(blk) * NBLK + p * 4096 + j;   // ys stride 64 for the direct update of y
     for (uint32_t b = a.add_start[blk]; b < a.add_start[blk + 1]; ++b) {
-        uint32_t const img = a.add_index[b];
-        int const L = a.image_lmax[img];
+        AddEntry const e = a.add_entry[b];
+        int const L = int(e.L);
         if (L <= 5) continue;   // added by the local kernel
         __syncthreads();        // H of the previous image is no longer read
-        stage_hermite<real_t>(H, xi2, a.htab, a.xtab, a.add_slot[b], L, tid, 512);
+        stage_hermite<real_t>(H, xi2, a.htab, a.xtab, e.slot, L, tid, 512);
         __syncthreads();
-        real_t const* dsrc = a.dcoef + (size_t(a.image_starts[img]) * a.nCols + col) * 128 + p * 64 + j;
+        real_t const* dsrc = a.dcoef + (size_t(e.coef0) * a.nCols + col) * 128 + p * 64 + j;
         sho_add_atom<real_t, 7>(yb, 64, H, xi2, R2, dsrc, size_t(a.nCols) * 128, zq, L);
     }
 }
@@ -1617,7 +1607,6 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
                 }
             }
         }
-        if ((s = dupload(d, d.nbr, nb, err))) return s;
 
         // the local kernels' block headers: block, column group, potential cell, add-list range, neighbour codes
         int const HW = (NH == 1) ? 16 : 32;
@@ -1641,7 +1630,6 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
 
     // dyadic structures
     if ((s = dupload(d, d.add_start, p.sho_add.start, err))) return s;
-    if ((s = dupload(d, d.add_index, p.sho_add.index, err))) return s;
     {   // blocks touched by atom images with numax > 5, whose add-back runs in a separate kernel after k_local
         std::vector<uint32_t> wide;
         for (uint64_t blk = 0; blk < p.nnzb; ++blk)
@@ -1657,28 +1645,20 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
         d.n_narrow_blocks = uint32_t(narrow.size());
         if ((s = dupload(d, d.narrow_blocks, narrow, err))) return s;
     }
-    {   // (image, target row) slots of the Hermite table, for the add lists and the projection chunks below
+    {   // add-list entries in one record each: image, numax, the slot of (image, target row) in the Hermite table
+        // (the projection chunks below add their own slots), first coefficient of the image
         std::map<uint64_t, uint32_t> slot_of;
-        std::vector<uint32_t> add_slot(p.sho_add.index.size());
-        for (uint64_t blk = 0; blk < p.nnzb; ++blk)
-            for (uint32_t k = p.sho_add.start[blk]; k < p.sho_add.start[blk + 1]; ++k) {
-                uint64_t const key = (uint64_t(p.sho_add.index[k]) << 32) | p.rowindx[blk];
-                auto const it = slot_of.emplace(key, uint32_t(slot_of.size())).first;
-                add_slot[k] = it->second;
-            }
-        d.slot_map = std::move(slot_of);
-        if ((s = dupload(d, d.add_slot, add_slot, err))) return s;
-    }
-    {   // add-list entries in one record each: image, numax, Hermite slot, first coefficient of the image
         std::vector<uint32_t> entry(4 * p.sho_add.index.size());
         for (uint64_t blk = 0; blk < p.nnzb; ++blk)
             for (uint32_t k = p.sho_add.start[blk]; k < p.sho_add.start[blk + 1]; ++k) {
                 uint32_t const img = p.sho_add.index[k];
+                uint64_t const key = (uint64_t(img) << 32) | p.rowindx[blk];
                 entry[4 * size_t(k) + 0] = img;
                 entry[4 * size_t(k) + 1] = uint32_t(p.image_lmax[img]);
-                entry[4 * size_t(k) + 2] = d.slot_map.at((uint64_t(img) << 32) | p.rowindx[blk]);
+                entry[4 * size_t(k) + 2] = slot_of.emplace(key, uint32_t(slot_of.size())).first->second;
                 entry[4 * size_t(k) + 3] = p.image_starts[img];
             }
+        d.slot_map = std::move(slot_of);
         if ((s = dupload(d, d.add_entry, entry, err))) return s;
     }
     if ((s = dupload(d, d.image_pos, p.image_pos, err))) return s;
@@ -1983,16 +1963,15 @@ int device_action(DevicePlan& d, ActionScalars const& s, real_t* y, real_t const
     }
     if (!(parts & ACTION_LOCAL)) return 0;
     LocalArgs<real_t> a;
-    a.y = y; a.x = x; a.dcoef = ws.dcoef; a.veff = d.veff; a.nbr = d.nbr; a.V = d.V;
-    a.add_start = d.add_start; a.add_index = d.add_index; a.colindx = d.colindx; a.rowindx = d.rowindx;
-    a.row_pos = d.row_pos; a.image_pos = d.image_pos; a.image_lmax = d.image_lmax; a.image_starts = d.image_starts;
-    a.nnzb = d.nnzb; a.nCols = d.nCols;
+    a.y = y; a.x = x; a.dcoef = ws.dcoef; a.V = d.V;
+    a.add_start = d.add_start; a.colindx = d.colindx;
+    a.nCols = d.nCols;
     a.zero = ws.zero;
     a.items = d.active_order ? d.active_items : d.items;
     a.add_entry = reinterpret_cast<AddEntry const*>(d.add_entry);
     a.nitems = d.active_order ? d.n_active : d.nnzb;
     a.E_col = d.E_col;
-    a.htab = d.htab; a.xtab = d.xtab; a.add_slot = d.add_slot;
+    a.htab = d.htab; a.xtab = d.xtab;
     a.dot = dot;
     int const engine = d.prj_engine[sizeof(real_t) == 8 ? 0 : 1];
     a.add_local = (engine == 1) ? 0 : 1;

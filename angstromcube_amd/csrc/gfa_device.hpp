@@ -61,7 +61,6 @@ struct DevicePlan {
     uint16_t* colindx = nullptr;       // [nnzb]
     uint32_t* rowindx = nullptr;       // [nnzb]
     int32_t* veff = nullptr;           // [nnzb]
-    int32_t* nbr = nullptr;            // [3][2][nh_max][nnzb] neighbour codes
     uint32_t* order = nullptr;         // [nnzb] processing order of k_local
     // per-block headers of the local kernels, head_words (16, or 32 when nh_max = 2) words each: the block, its
     // column group, potential cell, add-list range and neighbour codes [direction][side][k], so that a work-group
@@ -73,14 +72,12 @@ struct DevicePlan {
     // Hermite functions per (atom image, target row) slot: htab[slot][8][3][4], xtab[slot][3][4] = xi^2
     double* htab = nullptr;
     float* xtab = nullptr;
-    uint32_t* add_slot = nullptr;      // [add nnz] slot of each add-list entry
     uint32_t* prj_slot = nullptr;      // [projection blocks] slot of each chunk block
     uint32_t nslots = 0;
     std::map<uint64_t, uint32_t> slot_map;  // (image << 32 | row) -> slot, during upload only
     std::vector<uint32_t> h_order;     // host copy of order
     float* row_pos = nullptr;          // [nRows][4]
     uint32_t* add_start = nullptr;     // [nnzb+1]
-    uint32_t* add_index = nullptr;
     uint32_t* wide_blocks = nullptr;   // blocks touched by an atom image with numax > 5
     uint32_t n_wide_blocks = 0;
     uint32_t* narrow_blocks = nullptr; // blocks touched by an atom image with numax <= 5
