@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-halfsteps", type=int, default=10, help="half-steps of the timed CPU TFQMR sample")
+    ap.add_argument("--cpu-cores", type=int, default=16,
+                    help="host threads of the all-cores CPU sample (at most the logical CPUs; 1 skips it)")
     ap.add_argument("--budget-s", type=float, default=420.0,
                     help="wall seconds after which the optional batch-0 comparisons are skipped")
     ap.add_argument("--mode", choices=["bench", "action", "dyadic", "contour", "batch128", "batch"], default="bench",
@@ -523,6 +525,24 @@ def cpu_baseline(G, sysm, sample, half_steps, args):
     t_run = time.perf_counter() - t
     # the run applies the operator once per half-step, once to start and once for the final residual
     per_block_hs = t_run / ((m + 2) * nb)
+    # all cores: the same run on every host core at once, one per thread (the oracle's C solve releases the GIL
+    # and only reads the shared oracle state); throughput per block and half-step over all of them
+    ncores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
+    t_all = None
+    if ncores > 1:
+        import threading
+        gate = threading.Barrier(ncores + 1)
+        def run():
+            gate.wait()
+            o.tfqmr(tol=1e-30, maxit=nhs)
+        ths = [threading.Thread(target=run) for _ in range(ncores)]
+        for th in ths:
+            th.start()
+        gate.wait()
+        t = time.perf_counter()
+        for th in ths:
+            th.join()
+        t_all = time.perf_counter() - t
     p.close()
     nblocks = {}
     for d in sample:
@@ -531,12 +551,20 @@ def cpu_baseline(G, sysm, sample, half_steps, args):
             q = G.GreenPlan(sysm, r_trunc=r, fd_range=(4, 4, 4), sources=[SOURCE])
             nblocks[r] = q.nnzb
             q.close()
-    t_cpu = sum(per_block_hs * nblocks[d["radius"]] * (h + 2) for d, h in zip(sample, half_steps))
-    return {"value": round(64.0 * len(sample) / t_cpu, 6), "unit": "Gf columns/s", "cores": 1, "kind": "port",
-            "sample": f"oracle CPU TFQMR, {m} half-steps on the radius-{small['radius']:g} instance ({nb} blocks) in "
-                      f"{t_run:.1f} s (actions, vector updates and inner products), scaled by block count and by "
-                      f"the FP64 GPU solves' half-steps to the {len(sample)} instances of the plain FP64 sample; "
-                      f"host {platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs"}
+    work = sum(nblocks[d["radius"]] * (h + 2) for d, h in zip(sample, half_steps))   # blocks x half-steps
+    out = {"value": round(64.0 * len(sample) / (per_block_hs * work), 6), "unit": "Gf columns/s", "cores": 1,
+           "kind": "port",
+           "sample": f"oracle CPU TFQMR, {m} half-steps on the radius-{small['radius']:g} instance ({nb} blocks) in "
+                     f"{t_run:.1f} s (actions, vector updates and inner products), scaled by block count and by "
+                     f"the FP64 GPU solves' half-steps to the {len(sample)} instances of the plain FP64 sample; "
+                     f"host {platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs"}
+    if t_all is not None:
+        per_all = t_all / ((m + 2) * nb * ncores)
+        out["all_cores"] = {"value": round(64.0 * len(sample) / (per_all * work), 6), "unit": "Gf columns/s",
+                            "cores": ncores,
+                            "sample": f"the same run on {ncores} threads at once in {t_all:.1f} s (one per core, "
+                                      "as independent instances), scaled the same way"}
+    return out
 
 
 if __name__ == "__main__":
