@@ -1174,9 +1174,12 @@ __device__ __forceinline__ void load_lines(LocalArgs<real_t> const& a, int d, It
 }
 
 // PF: directions whose neighbour lines are issued before the first barrier (3: all; 2: x and y, z after the x
-// stencil; 1: x, then y after the x stencil and z after the y stencil) -- loads in flight against occupancy
+// stencil; 1: x, then y after the x stencil and z after the y stencil) -- loads in flight against occupancy.
+// The FP64 shape with 16 columns and PF 2 needs 130 VGPRs on its own; it is held to 128 (4 waves per SIMD instead
+// of 3, one 8-byte spill), which makes it 6% faster; with PF 3 the same bound spills heavily and is not applied.
 template <typename real_t, int NH, bool IMG, int JW, int PF>
-__global__ __launch_bounds__((JW / Lane<real_t>::n) * 32) void k_lines(LocalArgs<real_t> a) {
+__global__ __launch_bounds__((JW / Lane<real_t>::n) * 32, (sizeof(real_t) == 8 && JW == 16 && PF == 2) ? 4 : 1)
+void k_lines(LocalArgs<real_t> a) {
     using L16 = Lane<real_t>;
     constexpr int CPL = L16::n, LQ = JW / CPL, T = LQ * 32;
     constexpr int NSLICE = 64 / JW;

@@ -49,8 +49,9 @@ struct DevicePlan {
     bool has_images = false;           // some finite-difference halo is a periodic image
     int fd_range[3] = {4, 4, 4};
     // local-kernel tile shape per precision (index 0: FP64, 1: FP32), set by the autotuner; the defaults are the
-    // shapes measured fastest on a 159,256-block plan (FP64 k_local 16 columns x 2 lines, FP32 k_plane 32 columns)
-    int tune_kind[2] = {0, 1}, tune_jw[2] = {16, 32}, tune_lb[2] = {2, 0};
+    // shapes measured fastest on a 159,256-block plan (FP64 k_lines 16 columns, 2 directions ahead; FP32 k_plane
+    // 32 columns)
+    int tune_kind[2] = {2, 1}, tune_jw[2] = {16, 32}, tune_lb[2] = {2, 0};
     bool tuned[2] = {false, false};
     // SHO projection and add-back per precision: 0 the sum-factorised forms on the vector ALUs (the add-back inside
     // the local kernel), 1 the dense forms on the matrix cores (the add-back in its own kernel after the local one);

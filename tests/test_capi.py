@@ -41,10 +41,11 @@ def test_plan_without_gpu_and_queries():
 
 
 def test_default_and_pinned_tile_shapes_without_gpu():
-    """A new plan reports the default local-kernel shapes (FP64 k_local 16 columns x 2 lines, FP32 k_plane 32
-    columns) as not tuned; pinning records the shape as tuned, and unknown shapes are refused."""
+    """A new plan reports the default local-kernel shapes (FP64 k_lines with 16 columns and 2 directions ahead,
+    FP32 k_plane with 32 columns) as not tuned; pinning records the shape as tuned, and unknown shapes are
+    refused."""
     p = G.GreenPlan(S.small_periodic(), r_trunc=6.0)
-    assert p.tile_shape(64) == (0, 16, 2, 0) and p.tile_shape(32) == (1, 32, 0, 0)
+    assert p.tile_shape(64) == (2, 16, 2, 0) and p.tile_shape(32) == (1, 32, 0, 0)
     p.set_tile_shape(64, 1, 64)
     assert p.tile_shape(64) == (1, 64, 0, 1)
     assert p.dyadic_engine(32) == 0 and p.dyadic_engine(64) == 0
