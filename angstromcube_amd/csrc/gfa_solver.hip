@@ -1476,13 +1476,15 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
             if (int r = fetch_resid()) return r;
         }
     } else {
-        // FP32 inner solves inside FP64 refinement, per column group.  tfQMR: a group stays in FP32 while each FP32
-        // solve reduces its true (FP64) residual at least tenfold; a group whose FP32 solve gains less (the FP32
-        // recurrences of an ill-conditioned instance stall) is solved again in FP64 from zero.  COCG: a group that
-        // spends the FP32 budget without converging continues in FP64 from its FP32 iterate (COCG with residual
-        // replacement does not stall, but in FP32 it needs up to several times the FP64 iterations on
-        // ill-conditioned instances, and an FP64 iteration costs about twice an FP32 one).  The FP64 work vectors
-        // take the arena of the FP32 ones (the two never run at the same time).
+        // FP32 inner solves inside FP64 refinement, per column group.  tfQMR: one FP32 solve (to inner_tolerance or
+        // the FP32 budget), then a group that is not converged goes on in FP64 from its FP32 iterate; a group whose
+        // FP32 solve reduced its true (FP64) residual less than tenfold (the FP32 recurrences of an ill-conditioned
+        // instance stall) is solved in FP64 from zero instead.  Further FP32 refinement solves cost more than they
+        // gain with tfQMR: each restarts the Krylov space.  COCG: a group that spends the FP32 budget without
+        // converging continues in FP64 from its FP32 iterate (COCG with residual replacement does not stall, but in
+        // FP32 it needs up to several times the FP64 iterations on ill-conditioned instances, and an FP64 iteration
+        // costs about twice an FP32 one); otherwise it stays in FP32.  The FP64 work vectors take the arena of the
+        // FP32 ones (the two never run at the same time).
         double const floor_rel = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;  // gain asked of one FP32 solve
         int const budget32 = o.fp32_max_iterations > 0 ? o.fp32_max_iterations : o.max_iterations;
         std::vector<uint8_t> prec64(st.nCols, 0), done(st.nCols, 0);
@@ -1518,6 +1520,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
                     if (g <= o.tolerance) done[c] = 1;
                     else if (o.method == 1 && used >= budget32) { prec64[c] = 1; ++fp64_groups; }
                     else if (g > 0.1 * gres[c]) { prec64[c] = 1; fresh64[c] = 1; ++fp64_groups; }
+                    else if (o.method == 0) { prec64[c] = 1; ++fp64_groups; }
                     gres[c] = g;
                 }
                 if (o.echo > 0)

@@ -203,9 +203,10 @@ typedef struct {
     int32_t eager;              /* 0: replay each iteration from a captured HIP graph; 1: plain launches */
     int32_t autotune;           /* 1: time the action's tile shapes on the solver's vectors first (once per precision) */
     int32_t fp32_max_iterations;/* fp_bits 32: half-steps allowed to one FP32 solve (0: max_iterations).  tfQMR:
-                                 * a column group whose FP32 solve reduces its residual less than tenfold is
-                                 * solved again in FP64 from zero.  COCG: a group still unconverged when its FP32
-                                 * solve has used this budget continues in FP64 from the FP32 iterate            */
+                                 * after one FP32 solve an unconverged column group goes on in FP64 from its FP32
+                                 * iterate, or from zero if the FP32 solve reduced its residual less than tenfold.
+                                 * COCG: a group still unconverged when its FP32 solve has used this budget
+                                 * continues in FP64 from the FP32 iterate                                        */
     double* residuals;          /* optional host output [64 nCols]: final relative residual of every column          */
     int32_t* halfsteps;         /* optional host output [64 nCols]: operator applications of every column (all solves) */
     int32_t method;             /* inner Krylov method: 0 tfQMR (as the reference), 1 COCG (conjugate orthogonal CG;
