@@ -378,7 +378,7 @@ def action_mode(G, sysm, instances, n, args, stream, dev):
         t = time_action(p, 64, 2, args.action_reps, stream)
         log(f"# local fp64 shape {shape}: {t * 1e3:.3f} ms for {p.nnzb} blocks")
     p.set_tile_shape(32, 1, 32, 0)   # the default shapes again for the engine comparison
-    p.set_tile_shape(64, 0, 16, 2)
+    p.set_tile_shape(64, 2, 16, 2)
     dyadic_engines(p, args, stream)
     # a plain device copy of one FP64 vector (nnzb * 64 KiB each way): the calibration of HBM counters
     x = torch.randn((p.nnzb, 2, 64, 64), dtype=torch.float64, device=dev)
