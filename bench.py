@@ -327,7 +327,7 @@ def main():
         "higher_is_better": True,
         "scaling": "none (one GPU)",
         "vs_baseline": None,
-        "dtype": f"c32 {args.method} + f64 refinement (per-group FP64 fallback)",
+        "dtype": f"c32 {args.method} + f64 refinement (groups over the FP32 budget of {fp32_budget} go on in FP64)",
         "data": "synthetic (seeded Gaussian potential, random atom matrices, Gauss-Legendre contour)",
         "config": {"workload": "batched contour: 32^3-block cell, 128 atoms numax 5, 32 energies -1..0 Ha "
                                "(Im 0.005..0.3) x 4 k-points, radii {6,8,10,12} Bohr, one source block each; "
