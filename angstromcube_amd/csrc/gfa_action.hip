@@ -1578,7 +1578,6 @@ int device_upload_plan(Plan const& p, DevicePlan& d, std::string& err) {
         std::vector<uint32_t> order(p.nnzb);
         for (uint64_t b = 0; b < p.nnzb; ++b) order[b] = uint32_t(b);
         std::sort(order.begin(), order.end(), [&](uint32_t u, uint32_t v) { return key[u] < key[v]; });
-        if ((s = dupload(d, d.order, order, err))) return s;
         d.h_order = std::move(order);
     }
 

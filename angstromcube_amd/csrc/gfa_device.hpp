@@ -62,7 +62,6 @@ struct DevicePlan {
     uint16_t* colindx = nullptr;       // [nnzb]
     uint32_t* rowindx = nullptr;       // [nnzb]
     int32_t* veff = nullptr;           // [nnzb]
-    uint32_t* order = nullptr;         // [nnzb] processing order of k_local
     // per-block headers of the local kernels, head_words (16, or 32 when nh_max = 2) words each: the block, its
     // column group, potential cell, add-list range and neighbour codes [direction][side][k], so that a work-group
     // reads everything it needs about its block with one scalar load
@@ -76,7 +75,7 @@ struct DevicePlan {
     uint32_t* prj_slot = nullptr;      // [projection blocks] slot of each chunk block
     uint32_t nslots = 0;
     std::map<uint64_t, uint32_t> slot_map;  // (image << 32 | row) -> slot, during upload only
-    std::vector<uint32_t> h_order;     // host copy of order
+    std::vector<uint32_t> h_order;     // [nnzb] processing order of the local kernels (the headers follow it)
     float* row_pos = nullptr;          // [nRows][4]
     uint32_t* add_start = nullptr;     // [nnzb+1]
     uint32_t* wide_blocks = nullptr;   // blocks touched by an atom image with numax > 5
