@@ -865,14 +865,15 @@ __device__ __forceinline__ void plane_xy(LocalArgs<real_t> const& a, ItemHead<NH
 
 // The end of a register-plane kernel for one block slice: the SHO add-back of the atom images (numax <= 5)
 // touching the block (add-list entries b0 .. b1 - 1, column group col), the store of the thread's 16 points and,
-// if a.dot is set, the bilinear product (x, y) of every column of the slice.  cs is LDS of at least 2 * 64 * JW
-// elements that no thread reads any more (the coefficients of one image at a time, then the partial sums of the
-// product); xo = x of the other plane at the thread's points.  Contains work-group barriers: every thread of the
-// work-group calls it.
+// if a.dot is set, the bilinear product (x, y) of every column of the slice.  cs is LDS of at least 2 * 56 * JW
+// elements of its own (the coefficients of one image at a time, then the partial sums of the product); xsp and
+// xso point to this thread's column of x in the LDS copy of the slice (its plane and the other one), which stays
+// intact, so that x is not held in registers across the add-back.  Contains work-group barriers: every thread of
+// the work-group calls it.
 template <typename real_t, int JW>
 __device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_t blk, uint32_t col, uint32_t b0,
                                              uint32_t b1, int slice, int p, int zq, int jl, real_t (&acc)[16],
-                                             real_t const (&own)[16], real_t const (&xo)[16], real_t* cs,
+                                             real_t const* xsp, real_t const* xso, real_t* cs,
                                              real_t (*H)[3][4], float (*xi2)[4]) {
     constexpr int T = 8 * JW;
     using vec_t = typename std::conditional<sizeof(real_t) == 8, double2, float4>::type;
@@ -909,8 +910,8 @@ __device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_
         real_t re_t = 0, im_t = 0;   // 16 products in the vector precision, then summed in FP64
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            re_t += own[k] * acc[k];
-            im_t += xo[k] * acc[k];
+            re_t += xsp[(zq * 16 + k) * JW] * acc[k];
+            im_t += xso[(zq * 16 + k) * JW] * acc[k];
         }
         double re = re_t, im = im_t;
         if (p == 1) re = -re;
@@ -930,15 +931,17 @@ __device__ __forceinline__ void plane_finish(LocalArgs<real_t> const& a, uint64_
 
 // Register-plane variant of the local kernel: one work-group per (block, JW-column slice), 8 JW threads; thread
 // (jl, p, zq) owns the 16 points of z plane zq of column j = slice JW + jl, plane p, and accumulates potential,
-// kinetic x, y, z and the SHO add-back in registers.  LDS holds the block slice only (read by the z-stencil and
-// the imaginary part of the potential).  zq is uniform within a wave (2 JW >= 64), so the z-stencil is
-// specialised per plane.
+// kinetic x, y, z and the SHO add-back in registers.  LDS holds the block slice (read by the z-stencil, the
+// imaginary part of the potential and the bilinear product of the epilogue) and, apart, the coefficients of the
+// add-back, so that x need not stay in registers across the add-back (95 VGPRs instead of 124 in FP32).  zq is
+// uniform within a wave (2 JW >= 64), so the z-stencil is specialised per plane.
 template <typename real_t, int NH, bool IMG, int JW>
 __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
     static_assert(2 * JW >= 64, "zq must be uniform within a wave");
     constexpr int T = 8 * JW;
     constexpr int NSLICE = 64 / JW;
     __shared__ real_t xs[2 * 64 * JW];
+    __shared__ __align__(16) real_t cs[2 * 56 * JW];   // add-back coefficients (numax <= 5), then the (x, y) sums
     __shared__ real_t H[8][3][4];
     __shared__ float xi2[3][4];
 
@@ -1003,15 +1006,8 @@ __global__ __launch_bounds__(8 * JW) void k_plane(LocalArgs<real_t> a) {
         }
     }
 
-    // the other plane of x at this thread's points, for the bilinear product (x, y) of the epilogue, before the
-    // add-back takes over the LDS of the slice
-    real_t xo[16];
-    if (a.dot) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) xo[k] = xso[(zq * 16 + k) * JW];
-    }
-    plane_finish<real_t, JW>(a, blk, h.w[IH_COL], h.w[IH_ADD0], h.w[IH_ADD1], slice, p, zq, jl, acc, own, xo, xs, H,
-                             xi2);
+    plane_finish<real_t, JW>(a, blk, h.w[IH_COL], h.w[IH_ADD0], h.w[IH_ADD1], slice, p, zq, jl, acc, xsp, xso, cs,
+                             H, xi2);
 }
 
 // ---------------------------------------------------------------------------------------------
