@@ -508,6 +508,18 @@ def mfma_of(nnzb):
             for k, e in t.get("kernels", {}).items()}
 
 
+def cpu_model():
+    """The host CPU's model name (Linux), else the machine type."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(G, sysm, sample, half_steps, args):
     """The oracle's CPU TFQMR (a C restatement of the reference's action with the same TFQMR as the device: actions,
     vector updates and inner products), single thread, timed for --cpu-halfsteps half-steps on the radius-6
@@ -557,7 +569,7 @@ def cpu_baseline(G, sysm, sample, half_steps, args):
            "sample": f"oracle CPU TFQMR, {m} half-steps on the radius-{small['radius']:g} instance ({nb} blocks) in "
                      f"{t_run:.1f} s (actions, vector updates and inner products), scaled by block count and by "
                      f"the FP64 GPU solves' half-steps to the {len(sample)} instances of the plain FP64 sample; "
-                     f"host {platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs"}
+                     f"host {cpu_model()}, {os.cpu_count()} logical CPUs"}
     if t_all is not None:
         per_all = t_all / ((m + 2) * nb * ncores)
         out["all_cores"] = {"value": round(64.0 * len(sample) / (per_all * work), 6), "unit": "Gf columns/s",
