@@ -39,3 +39,21 @@ def test_every_profile_summary_names_its_command():
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.json"))):
         t = json.load(open(f))
         assert any("rocprofv3" in c for c in t["commands"]), f
+
+
+def test_cpu_baseline_single_and_all_cores():
+    """The bench's CPU baseline (the oracle's TFQMR on the host, one thread and several at once) runs on the CPU and
+    reports a positive rate for both, the all-cores one with its thread count."""
+    import types
+    from angstromcube_amd import gfa as G
+    from angstromcube_amd import synthetic as S
+    sysm = S.small_periodic()
+    inst = [{"E": complex(-0.5, 0.05), "k": (0.0, 0.0, 0.0), "radius": 6.0}]
+    save = B.SOURCE
+    B.SOURCE = (4, 4, 4)   # the centre block of the 8^3-block cell
+    try:
+        out = B.cpu_baseline(G, sysm, inst, [20], types.SimpleNamespace(cpu_halfsteps=2, cpu_cores=2))
+    finally:
+        B.SOURCE = save
+    assert out["value"] > 0 and out["cores"] == 1 and out["kind"] == "port"
+    assert out["all_cores"]["value"] > 0 and out["all_cores"]["cores"] == 2
