@@ -54,7 +54,9 @@ constexpr int SEG = SEG_BLOCKS;  // blocks per segment (work-group of the vector
 
 // per-column scalar fields, stored field-major: S[field * ncol + q]
 enum Field { RHO_RE, RHO_IM, ALPHA_RE, ALPHA_IM, THETA, TAU, ETAE_RE, ETAE_IM, ETAO_RE, ETAO_IM,
-             CDE_RE, CDE_IM, CDO_RE, CDO_IM, BETA_RE, BETA_IM, BNORM, STATUS, TOL, MSTEP, NFIELDS };
+             CDE_RE, CDE_IM, CDO_RE, CDO_IM, BETA_RE, BETA_IM, BNORM, STATUS, TOL, MSTEP,
+             APREV_RE, APREV_IM,   // COCG: alpha of the last even iteration, whose x update is deferred to the odd one
+             NFIELDS };
 enum Status { ACTIVE = 0, CONVERGED = 1, BREAKDOWN = 2 };
 
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -513,13 +515,44 @@ __global__ __launch_bounds__(256) void k_cocg_r(T* __restrict__ rv, T const* __r
     store_partials<T, 3>(acc, g.segpart, s);
 }
 
-// x += alpha p with the p of this iteration, then p = r + beta p.  x is the solution itself (FP64) or, for FP32
-// vectors, the FP32 increment since the last fold into the FP64 solution.
+// COCG keeps two search directions P0, P1 and updates x every second iteration: an even iteration (p in P0) forms
+// P1 = r + beta P0 and leaves x alone (its alpha is kept in APREV); the odd one (p in P1) adds both steps,
+// x += alpha_prev P0 + alpha P1, and forms P0 = r + beta P1.  That reads and writes x once per two iterations: 4.5
+// vector passes per iteration instead of 5.  x is the solution itself (FP64) or, for FP32 vectors, the FP32
+// increment since the last fold into the FP64 solution.
 template <typename T>
-__global__ __launch_bounds__(256) void k_cocg_px(T* __restrict__ x, T* __restrict__ p, T const* __restrict__ rv,
-                                                 double const* __restrict__ S, uint32_t ncol, SegArgs g) {
+__global__ __launch_bounds__(256) void k_cocg_p_even(T* __restrict__ pn, T const* __restrict__ po,
+                                                     T const* __restrict__ rv, double const* __restrict__ S,
+                                                     uint32_t ncol, SegArgs g) {
     SEG_PROLOGUE
-    Cx<T> a[V], be[V];
+    Cx<T> be[V];
+    coefs(be, S, ncol, q0, BETA_RE);
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint32_t const blk = g.col_blocks[b];
+#pragma unroll 2
+        for (int ps = 0; ps < M::NPASS; ++ps) {
+            int const i = ps * M::RPP + r;
+            T pr[V], pi[V], rr[V], ri[V];
+            ld(pr, po + OFF(0, i)); ld(pi, po + OFF(1, i));
+            ld(rr, rv + OFF(0, i)); ld(ri, rv + OFF(1, i));
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                T const nr = rr[t] + (be[t].re * pr[t] - be[t].im * pi[t]);
+                T const ni = ri[t] + (be[t].re * pi[t] + be[t].im * pr[t]);
+                pr[t] = nr; pi[t] = ni;
+            }
+            st(pn + OFF(0, i), pr); st(pn + OFF(1, i), pi);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cocg_px_odd(T* __restrict__ x, T* __restrict__ p0, T const* __restrict__ p1,
+                                                     T const* __restrict__ rv, double const* __restrict__ S,
+                                                     uint32_t ncol, SegArgs g) {
+    SEG_PROLOGUE
+    Cx<T> ap[V], a[V], be[V];
+    coefs(ap, S, ncol, q0, APREV_RE);
     coefs(a, S, ncol, q0, ALPHA_RE);
     coefs(be, S, ncol, q0, BETA_RE);
     for (uint32_t b = b0; b < b1; ++b) {
@@ -527,20 +560,23 @@ __global__ __launch_bounds__(256) void k_cocg_px(T* __restrict__ x, T* __restric
 #pragma unroll 2
         for (int ps = 0; ps < M::NPASS; ++ps) {
             int const i = ps * M::RPP + r;
-            T pr[V], pi[V], rr[V], ri[V], xr[V], xi[V];
-            ld(pr, p + OFF(0, i)); ld(pi, p + OFF(1, i));
+            T or_[V], oi[V], pr[V], pi[V], rr[V], ri[V], xr[V], xi[V];
+            ld(or_, p0 + OFF(0, i)); ld(oi, p0 + OFF(1, i));
+            ld(pr, p1 + OFF(0, i)); ld(pi, p1 + OFF(1, i));
             ld(rr, rv + OFF(0, i)); ld(ri, rv + OFF(1, i));
             ld(xr, x + OFF(0, i)); ld(xi, x + OFF(1, i));
 #pragma unroll
             for (int t = 0; t < V; ++t) {
+                xr[t] += ap[t].re * or_[t] - ap[t].im * oi[t];
+                xi[t] += ap[t].re * oi[t] + ap[t].im * or_[t];
                 xr[t] += a[t].re * pr[t] - a[t].im * pi[t];
                 xi[t] += a[t].re * pi[t] + a[t].im * pr[t];
                 T const nr = rr[t] + (be[t].re * pr[t] - be[t].im * pi[t]);
                 T const ni = ri[t] + (be[t].re * pi[t] + be[t].im * pr[t]);
-                pr[t] = nr; pi[t] = ni;
+                or_[t] = nr; oi[t] = ni;
             }
             st(x + OFF(0, i), xr); st(x + OFF(1, i), xi);
-            st(p + OFF(0, i), pr); st(p + OFF(1, i), pi);
+            st(p0 + OFF(0, i), or_); st(p0 + OFF(1, i), oi);
         }
     }
 }
@@ -838,13 +874,18 @@ __global__ __launch_bounds__(64 * FOLD_P) void k_cocg_alpha(double* __restrict__
 // COCG: after the r update: convergence on the recursive residual |r| <= tol; beta = (r', r') / (r, r)
 __global__ __launch_bounds__(64 * FOLD_P) void k_cocg_beta(double* __restrict__ S, double const* __restrict__ segpart,
                                                         uint32_t const* __restrict__ col_seg,
-                                                        uint8_t const* __restrict__ col_active, uint32_t ncol) {
+                                                        uint8_t const* __restrict__ col_active, uint32_t ncol,
+                                                        int save_alpha) {
     uint32_t const c = blockIdx.x, q = 64 * c + threadIdx.x;
     if (!col_active[c]) return;
     int const f[3] = {0, 1, 2};
     double t[3];
     fold_group<3>(segpart, col_seg, c, f, t);
     if (threadIdx.x >= 64) return;
+    if (save_alpha) {   // an even iteration: its x update is done by the next one (alpha 0 for stopped columns)
+        F(S, ncol, APREV_RE, q) = F(S, ncol, ALPHA_RE, q);
+        F(S, ncol, APREV_IM, q) = F(S, ncol, ALPHA_IM, q);
+    }
     if (F(S, ncol, STATUS, q) != ACTIVE) { F(S, ncol, BETA_RE, q) = 0; F(S, ncol, BETA_IM, q) = 0; return; }
     F(S, ncol, MSTEP, q) = F(S, ncol, MSTEP, q) + 1;
     double const rn = sqrt(t[0]);
@@ -1092,8 +1133,9 @@ struct ActiveScope {
 // work vector k of precision T in the arena
 template <typename T> T* work(SolverState& st, int k) { return reinterpret_cast<T*>(st.arena) + size_t(k) * st.nnzb * NBLK; }
 
-// work vectors per inner solve: TFQMR w, u, un, v, d, A un, A u; COCG r, p, A p and, in FP32, the FP32 increment of x
-template <typename T> int work_vectors(int method) { return method == 1 ? (sizeof(T) == 4 ? 4 : 3) : 7; }
+// work vectors per inner solve: TFQMR w, u, un, v, d, A un, A u; COCG r, the two search directions, A p and, in
+// FP32, the FP32 increment of x
+template <typename T> int work_vectors(int method) { return method == 1 ? (sizeof(T) == 4 ? 5 : 4) : 7; }
 
 // an arena of at least n work vectors of T; a larger one replaces the old (whose contents are not kept: every
 // inner solve starts from its right-hand side), and a captured graph that points into it is dropped
@@ -1148,18 +1190,19 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
     int m = 0;
     while (m < maxit) {
         int const nsteps = std::min(steps_per_check, (maxit - m + halfsteps_per_step - 1) / halfsteps_per_step);
-        if (eager) {
+        // step k of the interval (intervals start at multiples of steps_per_check: COCG uses the step's parity)
+        if (eager || nsteps != steps_per_check) {   // a shorter last interval runs its steps directly
             for (int k = 0; k < nsteps; ++k)
-                if (int r = step(s)) return r;
+                if (int r = step(s, k)) return r;
         } else {
-            // a full poll interval is one graph of nsteps steps; a shorter last interval replays a one-step graph
-            int const gsteps = (nsteps == steps_per_check) ? nsteps : 1;
+            // a full poll interval is one graph of nsteps steps
+            int const gsteps = nsteps;
             if (!st.graph || st.graph_fp != fp || st.graph_seed != key || st.graph_steps != gsteps) {
                 drop_graph(st);
                 hipGraph_t gr = nullptr;
                 HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 int r = 0;
-                for (int k = 0; k < gsteps && !r; ++k) r = step(s);
+                for (int k = 0; k < gsteps && !r; ++k) r = step(s, k);
                 hipError_t const e = hipStreamEndCapture(s, &gr);
                 if (r) { if (gr) (void)hipGraphDestroy(gr); return r; }
                 if (e != hipSuccess) { err = std::string("graph capture: ") + hipGetErrorString(e); return GFA_ERR_DEVICE; }
@@ -1237,12 +1280,15 @@ int tfqmr(gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, 
     }
 
     return iterate(plan, st, scope, maxit, std::max(1, check_every / 2), 2, fp, seed * 2 + 0, eager,
-                   [&](hipStream_t ss) { return enqueue_step<T>(plan, st, V, seed, ss, err); }, res, err);
+                   [&](hipStream_t ss, int) { return enqueue_step<T>(plan, st, V, seed, ss, err); }, res, err);
 }
 
-// one COCG iteration: q = A p, alpha, x and r, beta, p
+// one COCG iteration of parity `odd`: q = A p (p in P1 if odd, else P0), alpha, r, beta, then P1 = r + beta P0
+// (even) or x += alpha_prev P0 + alpha P1, P0 = r + beta P1 (odd)
 template <typename T>
-int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p, T* q, hipStream_t s, std::string& err) {
+int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p0, T* p1, T* q, int odd, hipStream_t s,
+                 std::string& err) {
+    T* const p = odd ? p1 : p0;
     int const fp = sizeof(T) == 8 ? 64 : 32;
     SegArgs const g{st.act_seg, st.seg, st.col_blocks, st.segpart};
     dim3 const gb(st.nseg), tb(256), gs(st.nCols), ts(64 * FOLD_P);
@@ -1255,8 +1301,9 @@ int enqueue_cocg(gfa_plan* plan, SolverState& st, T* x, T* r, T* p, T* q, hipStr
     else hipLaunchKernelGGL((k_cocg_pq<T>), gb, tb, 0, s, p, q, g);
     hipLaunchKernelGGL(k_cocg_alpha, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
     hipLaunchKernelGGL((k_cocg_r<T>), gb, tb, 0, s, r, q, st.S, st.ncol, g);
-    hipLaunchKernelGGL(k_cocg_beta, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol);
-    hipLaunchKernelGGL((k_cocg_px<T>), gb, tb, 0, s, x, p, r, st.S, st.ncol, g);
+    hipLaunchKernelGGL(k_cocg_beta, gs, ts, 0, s, st.S, st.segpart, st.col_seg, st.col_active, st.ncol, odd ? 0 : 1);
+    if (odd) hipLaunchKernelGGL((k_cocg_px_odd<T>), gb, tb, 0, s, x, p0, p1, r, st.S, st.ncol, g);
+    else hipLaunchKernelGGL((k_cocg_p_even<T>), gb, tb, 0, s, p1, p0, r, st.S, st.ncol, g);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1268,10 +1315,11 @@ template <typename T>
 int cocg(gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, double const* tol_abs, int maxit,
          int check_every, bool eager, bool autotune, TfqmrResult& res, std::string& err) {
     if (int s = ensure_work<T>(st, work_vectors<T>(1), err)) return s;
-    // r, p, q; x = the FP32 increment of the solution (FP32) or the FP64 solution itself
+    // r, the search directions P0 and P1, q; x = the FP32 increment of the solution (FP32) or the FP64 solution
     T *r = work<T>(st, 0), *p = work<T>(st, 1), *q = work<T>(st, 2);
     bool const f32 = sizeof(T) == 4;
     T* x = f32 ? work<T>(st, 3) : reinterpret_cast<T*>(st.x64);
+    T* const p1 = work<T>(st, f32 ? 4 : 3);
     int const fp = f32 ? 32 : 64;
     hipStream_t const s = st.stream;
     DevicePlan& dev = capi_dev(plan);
@@ -1317,9 +1365,13 @@ int cocg(gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, d
         HIP_TRY(hipGetLastError());
         return 0;
     };
-    int const rc = iterate(plan, st, scope, maxit, std::max(1, check_every), 1, fp, 1, eager,
-                           [&](hipStream_t ss) { return enqueue_cocg<T>(plan, st, x, r, p, q, ss, err); }, res, err,
-                           hook);
+    // an even number of iterations per poll interval and in all (an odd limit runs one iteration more), so that
+    // every interval starts with an even iteration and ends with an odd one: no x update is left deferred when
+    // the host polls, compacts the active set or returns
+    int const per_check = std::max(2, (check_every + 1) & ~1);
+    int const rc = iterate(plan, st, scope, maxit + (maxit & 1), per_check, 1, fp, 1, eager,
+                           [&](hipStream_t ss, int k) { return enqueue_cocg<T>(plan, st, x, r, p, p1, q, k & 1, ss, err); },
+                           res, err, hook);
     if (rc) return rc;
     if (f32) {   // the increments of every group that took part, also of those retired since the last fold
         if (int e = set_active(plan, st, started, err)) return e;
