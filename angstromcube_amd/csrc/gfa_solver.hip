@@ -1542,6 +1542,9 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
         std::vector<uint8_t> prec64(st.nCols, 0), done(st.nCols, 0);
         std::vector<double> gres(st.nCols, 1.0);  // max relative residual of each group (||b|| = 1)
         bool first32 = true;
+        // the arena at the size of the FP64 inner solve from the start (which also holds the FP32 vectors), so that
+        // a group going on in FP64 does not free and allocate it again in the middle of the solve
+        if (int r = ensure_work<double>(st, work_vectors<double>(o.method), err)) return r;
         auto group_max = [&](uint32_t c) { double m = 0; for (int j = 0; j < 64; ++j) m = std::max(m, resid[64 * c + j]); return m; };
         for (int k = 0; k <= o.max_refinements; ++k) {
             std::vector<uint8_t> a32(st.nCols, 0), a64(st.nCols, 0), fresh64(st.nCols, 0);
