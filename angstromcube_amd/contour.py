@@ -47,7 +47,7 @@ def energy_mesh(kBT=1e-2, eBot=-1.0, nBot=5, nPar=15, nFer=9, nPol=3):
 def contour_density(system, r_trunc, E, wE, kpoints=((0.0, 0.0, 0.0, 1.0),), E_Fermi=0.0, nsources=(1, 1, 1),
                     sources=None, fd_range=(4, 4, 4), r_proj=6.0, max_groups=64, dVol=None, fp_bits=64,
                     tolerance=1e-9, max_iterations=4000, max_refinements=4, method="tfqmr", inner_tolerance=0.0,
-                    fp32_max_iterations=0, autotune=False):
+                    fp32_max_iterations=0, autotune=False, fp32_progress_check=0):
     """rho[nsources][64] = Re sum_E w_E sum_k w_k G_ii / pi and the reference's response density, for the source
     blocks of the description (a box of nsources at the cell centre, or an explicit list), the system's potential
     and raw atom matrices; returns (rho, rho_response, stats)."""
@@ -71,6 +71,7 @@ def contour_density(system, r_trunc, E, wE, kpoints=((0.0, 0.0, 0.0, 1.0),), E_F
     s.fp_bits, s.max_iterations, s.tolerance, s.max_refinements = fp_bits, max_iterations, tolerance, max_refinements
     s.check_every, s.inner_tolerance, s.autotune = 8, inner_tolerance, 1 if autotune else 0
     s.fp32_max_iterations = fp32_max_iterations
+    s.fp32_progress_check = fp32_progress_check
     s.method = {"tfqmr": 0, "cocg": 1}[method]
     nsrc = len(np.asarray(sources).reshape(-1, 3)) if sources is not None else None
     if nsrc is None:   # the box of source blocks, as the plan builder places it

@@ -56,6 +56,7 @@ constexpr int SEG = SEG_BLOCKS;  // blocks per segment (work-group of the vector
 enum Field { RHO_RE, RHO_IM, ALPHA_RE, ALPHA_IM, THETA, TAU, ETAE_RE, ETAE_IM, ETAO_RE, ETAO_IM,
              CDE_RE, CDE_IM, CDO_RE, CDO_IM, BETA_RE, BETA_IM, BNORM, STATUS, TOL, MSTEP,
              APREV_RE, APREV_IM,   // COCG: alpha of the last even iteration, whose x update is deferred to the odd one
+             RN,                   // COCG: the recursive residual norm of the last iteration relative to |b|
              NFIELDS };
 enum Status { ACTIVE = 0, CONVERGED = 1, BREAKDOWN = 2 };
 
@@ -834,6 +835,7 @@ __global__ __launch_bounds__(64 * FOLD_P) void k_cocg_scalar_init(double* __rest
         if (threadIdx.x < 64) {
             F(S, ncol, STATUS, q) = CONVERGED;
             F(S, ncol, MSTEP, q) = 0;
+            F(S, ncol, RN, q) = 0;
             freeze_step(S, ncol, q);
         }
         return;
@@ -851,6 +853,7 @@ __global__ __launch_bounds__(64 * FOLD_P) void k_cocg_scalar_init(double* __rest
     F(S, ncol, MSTEP, q) = 0;
     freeze_step(S, ncol, q);
     F(S, ncol, STATUS, q) = (bn <= tol) ? CONVERGED : ACTIVE;
+    F(S, ncol, RN, q) = (bn <= tol) ? 0.0 : 1.0;
 }
 
 // COCG: alpha = (r, r) / (p, A p)
@@ -889,6 +892,7 @@ __global__ __launch_bounds__(64 * FOLD_P) void k_cocg_beta(double* __restrict__ 
     if (F(S, ncol, STATUS, q) != ACTIVE) { F(S, ncol, BETA_RE, q) = 0; F(S, ncol, BETA_IM, q) = 0; return; }
     F(S, ncol, MSTEP, q) = F(S, ncol, MSTEP, q) + 1;
     double const rn = sqrt(t[0]);
+    F(S, ncol, RN, q) = rn / F(S, ncol, BNORM, q);   // relative to the right-hand side (> 0 for an active column)
     double const rr = F(S, ncol, RHO_RE, q), ri = F(S, ncol, RHO_IM, q), den = rr * rr + ri * ri;
     if (rn <= F(S, ncol, TOL, q)) {
         F(S, ncol, STATUS, q) = CONVERGED;   // x already holds this step
@@ -999,6 +1003,12 @@ struct SolverState {
     size_t held_bytes = 0;                  // device bytes of allocations[]
     // per solve: operator applications after which a column group leaves (empty or 0: no cap), and counters
     std::vector<int32_t> group_cap;
+    // per FP32 solve: the application count of the progress check (0: none), the residual a group must be below
+    // by then, the groups that failed it (they leave the solve at once: retire) and the groups that failed it
+    // in this solve (lagging, kept for the precision decision after the solve)
+    int progress_check = 0;
+    double progress_floor = 0;
+    std::vector<uint8_t> retire, lagging;
     int host_syncs = 0, graph_captures = 0;
 };
 
@@ -1231,6 +1241,7 @@ int iterate(gfa_plan* plan, SolverState& st, ActiveScope& scope, int maxit, int 
             bool live = false;
             for (int j = 0; j < 64 && !live; ++j) live = (st.S_host[64 * c + j] == ACTIVE);
             if (live && !st.group_cap.empty() && st.group_cap[c] > 0 && m >= st.group_cap[c]) { live = false; capped = true; }
+            if (live && !st.retire.empty() && st.retire[c]) { live = false; capped = true; }
             if (live) { any = true; live_blocks += st.h_cstart[c + 1] - st.h_cstart[c]; }
             else { act[c] = 0; changed = true; }
         }
@@ -1352,7 +1363,24 @@ int cocg(gfa_plan* plan, SolverState& st, double const* rhs_y, double tol_rel, d
     constexpr int REPLACE = 64;
     int last = 0;
     double* y64 = st.y64;   // A x64, replaced for the active groups (their right-hand side is no longer read)
+    bool checked = false;
     auto hook = [&](int m) -> int {
+        if (f32 && st.progress_check > 0 && !checked && m >= st.progress_check) {
+            // the progress check: a group whose recursive residual (max over its columns) is still above the
+            // floor leaves this solve at this poll and is marked for FP64
+            checked = true;
+            std::vector<double> rn(st.ncol);
+            HIP_TRY(hipMemcpyAsync(rn.data(), st.S + size_t(RN) * st.ncol, st.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            ++st.host_syncs;
+            st.retire.assign(st.nCols, 0);
+            for (uint32_t c = 0; c < st.nCols; ++c) {
+                if (!st.active[c]) continue;
+                double g = 0;
+                for (int j = 0; j < 64; ++j) g = std::max(g, rn[64 * c + j]);
+                if (g > st.progress_floor) st.retire[c] = st.lagging[c] = 1;
+            }
+        }
         if (!f32 || m - last < REPLACE) return 0;
         last = m;
         if (int e = fold()) return e;
@@ -1558,8 +1586,15 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
                 if (int r = set_active(plan, st, a32, err)) return r;
                 std::vector<double> const hs0 = st.col_halfsteps;   // applications before this FP32 solve
                 TfqmrResult tr;
-                if (int r = krylov<float>(o.method, plan, st, first32 ? nullptr : st.y64, floor_rel, st.tol_abs, budget32,
-                                         check_every, seed + k, eager, o.autotune != 0, tr, err)) return r;
+                st.lagging.assign(st.nCols, 0);
+                st.retire.clear();
+                st.progress_check = (o.method == 1 && first32) ? std::max(0, o.fp32_progress_check) : 0;
+                st.progress_floor = std::cbrt(o.tolerance);
+                int const rk = krylov<float>(o.method, plan, st, first32 ? nullptr : st.y64, floor_rel, st.tol_abs,
+                                             budget32, check_every, seed + k, eager, o.autotune != 0, tr, err);
+                st.progress_check = 0;
+                st.retire.clear();
+                if (rk) return rk;
                 first32 = false;
                 S.iterations += tr.halfsteps; S.actions += tr.actions;
                 if (int r = set_active(plan, st, a32, err)) return r;
@@ -1573,7 +1608,7 @@ int solve(gfa_plan* plan, gfa_solve_opts const& o, double* rho, gfa_solve_stats*
                     for (int j = 0; j < 64; ++j) used = std::max(used, st.col_halfsteps[64 * c + j] - hs0[64 * c + j]);
                     mx = std::max(mx, g);
                     if (g <= o.tolerance) done[c] = 1;
-                    else if (o.method == 1 && used >= budget32) { prec64[c] = 1; ++fp64_groups; }
+                    else if (o.method == 1 && (used >= budget32 || st.lagging[c])) { prec64[c] = 1; ++fp64_groups; }
                     else if (g > 0.1 * gres[c]) { prec64[c] = 1; fresh64[c] = 1; ++fp64_groups; }
                     else if (o.method == 0) { prec64[c] = 1; ++fp64_groups; }
                     gres[c] = g;

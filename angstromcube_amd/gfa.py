@@ -59,7 +59,7 @@ class SolveOpts(C.Structure):
                 ("X_device", C.c_void_p), ("stream", C.c_void_p), ("inner_tolerance", C.c_double),
                 ("echo", C.c_int32), ("eager", C.c_int32), ("autotune", C.c_int32),
                 ("fp32_max_iterations", C.c_int32), ("residuals", C.POINTER(C.c_double)), ("halfsteps", C.POINTER(C.c_int32)),
-                ("method", C.c_int32), ("reserved", C.c_int32), ("group_max_iterations", C.POINTER(C.c_int32))]
+                ("method", C.c_int32), ("fp32_progress_check", C.c_int32), ("group_max_iterations", C.POINTER(C.c_int32))]
 
 
 class ContourOpts(C.Structure):
@@ -410,7 +410,7 @@ class GreenPlan:
 
     def solve(self, fp_bits=64, tolerance=1e-9, max_iterations=400, max_refinements=4, check_every=8,
               shadow_seed=0, X=None, stream=None, inner_tolerance=0.0, echo=0, eager=False, autotune=False, fp32_max_iterations=0, method="tfqmr",
-              group_max_iterations=None):
+              group_max_iterations=None, fp32_progress_check=0):
         """Solve (H - E S) X = B; returns (rho[nCols, 64] complex, stats dict)."""
         import torch
         o = SolveOpts()
@@ -425,6 +425,7 @@ class GreenPlan:
         o.eager = 1 if eager else 0
         o.autotune = 1 if autotune else 0
         o.fp32_max_iterations = fp32_max_iterations
+        o.fp32_progress_check = fp32_progress_check
         o.method = {"tfqmr": 0, "cocg": 1}[method]
         o.X_device = C.c_void_p(X.data_ptr()) if X is not None else None
         caps = None

@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--fp32-maxit", type=int, default=0,
                     help="operator applications allowed to one FP32 solve (0: 1200 for COCG, whose column groups then "
                          "go on in FP64, 3000 for tfQMR)")
+    ap.add_argument("--fp32-check", type=int, default=-1,
+                    help="COCG: applications after which column groups whose residual is still above 1e-3 leave the "
+                         "FP32 solve for FP64 (-1: 2/5 of the FP32 budget; 0: no check)")
     ap.add_argument("--method", choices=["cocg", "tfqmr"], default="cocg",
                     help="inner Krylov method of the mixed solve")
     ap.add_argument("--inner-tol", type=float, default=0.0,
@@ -156,8 +159,9 @@ def main():
 
     inner = args.inner_tol if args.inner_tol > 0 else (1e-30 if args.method == "cocg" else 1e-4)
     fp32_budget = args.fp32_maxit or (1200 if args.method == "cocg" else 3000)
+    fp32_check = 0 if args.method != "cocg" else (args.fp32_check if args.fp32_check >= 0 else 2 * fp32_budget // 5)
     opts = dict(tolerance=1e-9, max_iterations=args.maxit, max_refinements=6, inner_tolerance=inner,
-                fp32_max_iterations=fp32_budget, method=args.method)
+                fp32_max_iterations=fp32_budget, fp32_progress_check=fp32_check, method=args.method)
 
     if args.mode == "batch128":
         batch128_mode(G, sysm, instances, opts)
@@ -327,7 +331,9 @@ def main():
         "higher_is_better": True,
         "scaling": "none (one GPU)",
         "vs_baseline": None,
-        "dtype": f"c32 {args.method} + f64 refinement (groups over the FP32 budget of {fp32_budget} go on in FP64)",
+        "dtype": f"c32 {args.method} + f64 refinement (groups over the FP32 budget of {fp32_budget}"
+                 + (f", or above residual 1e-3 after {fp32_check} applications," if fp32_check else "")
+                 + " go on in FP64)",
         "data": "synthetic (seeded Gaussian potential, random atom matrices, Gauss-Legendre contour)",
         "config": {"workload": "batched contour: 32^3-block cell, 128 atoms numax 5, 32 energies -1..0 Ha "
                                "(Im 0.005..0.3) x 4 k-points, radii {6,8,10,12} Bohr, one source block each; "

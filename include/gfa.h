@@ -213,7 +213,10 @@ typedef struct {
                                  * needs a complex-symmetric operator: real Bloch phases, e.g. Gamma or a wrapped
                                  * cell; one application per iteration, no squared residual polynomial, so its FP32
                                  * recurrences keep converging where tfQMR's stall)                                 */
-    int32_t reserved;
+    int32_t fp32_progress_check;/* fp_bits 32, COCG: after this many applications of the first FP32 solve (0: no
+                                 * check), a column group whose recursive residual is still above tolerance^(1/3)
+                                 * (less than a third of the way to the tolerance, in decades) leaves the FP32
+                                 * solve and goes on in FP64 from its FP32 iterate, as at the FP32 budget           */
     const int32_t* group_max_iterations; /* optional [nCols]: a column group leaves the solve after this many
                                  * operator applications (0: no cap); with a tolerance far below reach every group
                                  * then stops at a fixed iterate (solves of nearby operators stop at the same

@@ -159,6 +159,33 @@ def test_cocg_groups_over_the_fp32_budget_go_on_in_fp64(device, name, mk, kw, E,
 
 
 @pytest.mark.parametrize("name,mk,kw,E,k", [c for c in CASES if c[4] == (0.0, 0.0, 0.0)])
+def test_cocg_groups_failing_the_progress_check_go_on_in_fp64(device, name, mk, kw, E, k):
+    """Mixed COCG with a progress check after 8 FP32 applications and a budget no group reaches: the groups whose
+    recursive residual is still above 1e-9^(1/3) then leave FP32 and go on in FP64 from their FP32 iterate; without
+    the check every group stays in FP32.  Both solves meet the residual bar (oracle action) and give the FP64
+    solution."""
+    plan = G.GreenPlan(mk(), **kw)
+    plan.set_inputs(E, k)
+    X64, _, s64 = run(plan, device, fp_bits=64, tolerance=1e-9, max_iterations=2000, max_refinements=4, method="cocg")
+    ora = O.Oracle(plan, E, k)
+    for check in (0, 8):
+        Xb, _, sb = run(plan, device, fp_bits=32, tolerance=1e-9, max_iterations=2000, max_refinements=6,
+                        method="cocg", inner_tolerance=1e-30, fp32_max_iterations=2000, fp32_progress_check=check)
+        r = true_residuals(plan, ora, Xb)
+        err = np.abs(Xb - X64).max() / np.abs(X64).max()
+        print(f"{name}, progress check {check}: {sb['fp64_groups']} of {plan.nCols} groups went on in FP64, "
+              f"{sb['iterations']} applications (fp64 alone {s64['iterations']}), residual {r.max():.2e}, "
+              f"|X - X64| / |X64| = {err:.2e}")
+        if check:
+            assert sb["fp64_groups"] >= 1
+        else:
+            assert sb["fp64_groups"] == 0
+        assert sb["converged_columns"] == sb["total_columns"]
+        assert r.max() <= 1e-9
+        assert err < 1e-6
+
+
+@pytest.mark.parametrize("name,mk,kw,E,k", [c for c in CASES if c[4] == (0.0, 0.0, 0.0)])
 def test_cocg_solves_complex_symmetric_systems(device, name, mk, kw, E, k):
     """COCG (Gamma: A^T = A) in FP64 and in FP32 with FP64 refinement reaches the tolerance, measured with the
     oracle's action, and gives the tfQMR solution."""
