@@ -164,6 +164,11 @@ public:
     int max_refinements = 4;     // FP64 refinement steps of an FP32 solve
     int method = 0;              // inner Krylov method: 0 tfQMR (as the reference), 1 COCG
     double tolerance = 1e-9;     // tfqmrgpu::solve(action, buffer, 1e-9, ...) (include/green_action.hxx:216)
+    // the FP32 policy of gfa_solve_opts (0: the library's defaults): residual reduction asked of one FP32 solve
+    // (COCG: 1e-30, one FP32 solve with residual replacement), the FP32 budget and the progress check
+    double inner_tolerance = 0;
+    int fp32_max_iterations = 0;
+    int fp32_progress_check = 0;
 
     // rho[plan.nCols][64] = G_ii / pi of the diagonal blocks (include/green_action.hxx:226-238).
     // iterations > 0: solve with at most that many iterations; 0: only check the construction (returns 0);
@@ -213,6 +218,9 @@ private:
         o.tolerance = tolerance;
         o.max_refinements = max_refinements;
         o.method = method;
+        o.inner_tolerance = inner_tolerance;
+        o.fp32_max_iterations = fp32_max_iterations;
+        o.fp32_progress_check = fp32_progress_check;
         return o;
     }
     plan_t* p_;

@@ -149,6 +149,20 @@ int main(int argc, char** argv) {
         gfa::green_solver_t noco(&plan, 0, 0, fp, 2);
         std::fprintf(info, "key %d status %d\n", noco.action_key(), noco.solve(rho.data(), plan.nCols, 2000));
     }
+    {   // key 32021 with the mixed COCG policy, set through the solver's public members: one FP32 solve with
+        // residual replacement, groups still above 1e-3 after 8 FP32 applications go on in FP64
+        gfa::green_solver_t solver(&plan, 0, 0, 32, 1);
+        solver.method = 1;
+        solver.inner_tolerance = 1e-30;
+        solver.max_refinements = 6;
+        solver.fp32_max_iterations = 1200;
+        solver.fp32_progress_check = 8;
+        gfa_solve_stats st{};
+        int const s = solver.solve(rho.data(), plan.nCols, 2000, 1, 0, &st);
+        std::fprintf(info, "cocg32 status %d converged %d of %d max_residual %.3e fp64_groups %d\n", s,
+                     st.converged_columns, st.total_columns, st.max_residual, st.fp64_groups);
+        write_all(dir + "/rho32cocg.bin", rho.data(), rho.size());
+    }
     {   // iterations 0: construction check only; < 0: the reference's action timing loop
         gfa::green_solver_t solver(&plan, 0, 0, 64, 1);
         int const s0 = solver.solve(rho.data(), plan.nCols, 0);

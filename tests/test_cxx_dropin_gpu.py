@@ -3,7 +3,8 @@
 The program builds the plan of the 8^3-block periodic cell (4 atoms numax 3, Gamma, E = -0.5 + 0.05i Ha, one
 source block, radius 6 Bohr) through gfa::plan_t, applies gfa::action_t<double> and gfa::action_t<float>
 ::multiply with the plan's colindx in device memory (the call tfQMRgpu makes, include/green_action.hxx:140-149),
-and solves with gfa::green_solver_t for the action keys 64021 and 32021 (src/green_solver.cxx:31-51).  Checked
+and solves with gfa::green_solver_t for the action keys 64021 and 32021 (src/green_solver.cxx:31-51), the latter
+also with the mixed COCG policy (FP32 budget, progress check) set through the solver's public members.  Checked
 here: the actions against the oracle (1e-12 FP64, 2e-5 FP32), the densities against the FP64 solution the oracle's
 CPU TFQMR reaches with the same shadow residual, the refused keys 64022/32022 and a colIndex that differs from
 the plan's."""
@@ -93,7 +94,10 @@ def test_drop_in_action_and_solver_on_device(tmp_path):
     Xo, _, m_o = ora.tfqmr(tol=1e-12, maxit=4000, seed=SEED)
     sub = plan.export("SUBSET")
     rho_o = np.stack([np.diag(Xo[b, 0]) + 1j * np.diag(Xo[b, 1]) for b in sub]).reshape(-1) / np.pi
-    for name in ("rho64", "rho32"):
+    f = info["cocg32"].split()   # mixed COCG through the solver's FP32 policy members
+    assert f[2] == "0" and f[4] == f[6] and float(f[8]) <= 1e-9, info["cocg32"]
+    assert int(f[10]) >= 1, info["cocg32"]   # groups failed the progress check after 8 FP32 applications
+    for name in ("rho64", "rho32", "rho32cocg"):
         rho = np.fromfile(os.path.join(tmp_path, name + ".bin"), dtype=np.complex128)
         err = np.abs(rho - rho_o).max() / np.abs(rho_o).max()
         print(f"{name} vs oracle TFQMR (tolerance 1e-12, {m_o} half-steps): {err:.2e}")
