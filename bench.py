@@ -123,6 +123,15 @@ def rel_error(A, B):
     return float((A - B).abs().max() / B.abs().max()), d
 
 
+def fp32_policy(args):
+    """(FP32 budget, progress check) of the mixed solve: COCG 1200 applications and a check at 2/5 of the budget,
+    tfQMR 3000 and no check, unless given on the command line."""
+    budget = args.fp32_maxit or (1200 if args.method == "cocg" else 3000)
+    if args.method != "cocg":
+        return budget, 0
+    return budget, (args.fp32_check if args.fp32_check >= 0 else 2 * budget // 5)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -158,8 +167,7 @@ def main():
         return
 
     inner = args.inner_tol if args.inner_tol > 0 else (1e-30 if args.method == "cocg" else 1e-4)
-    fp32_budget = args.fp32_maxit or (1200 if args.method == "cocg" else 3000)
-    fp32_check = 0 if args.method != "cocg" else (args.fp32_check if args.fp32_check >= 0 else 2 * fp32_budget // 5)
+    fp32_budget, fp32_check = fp32_policy(args)
     opts = dict(tolerance=1e-9, max_iterations=args.maxit, max_refinements=6, inner_tolerance=inner,
                 fp32_max_iterations=fp32_budget, fp32_progress_check=fp32_check, method=args.method)
 
